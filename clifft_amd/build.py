@@ -1,0 +1,56 @@
+"""Build recipe for the native libraries (in-tree, gfx950 only).
+
+    python -m clifft_amd.build          # builds clifft_amd/libclifft_svm.so
+
+The library is compiled with hipcc for gfx950 with packed-FP32 instruction
+selection disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+LIB_SVM = os.path.join(HERE, "libclifft_svm.so")
+SVM_SOURCES = ["svm_kernels.hip", "svm_host.cpp"]
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-ffp-contract=fast",
+]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_svm(verbose: bool = True) -> str:
+    objs = []
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    for src in SVM_SOURCES:
+        obj = os.path.join(HERE, "_build", src + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        _run([HIPCC, *HIP_FLAGS, *lang, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB_SVM + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB_SVM)
+    return LIB_SVM
+
+
+def build_all() -> None:
+    build_svm()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

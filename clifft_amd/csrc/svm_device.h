@@ -1,0 +1,219 @@
+// svm_device.h -- device-side data layout and the per-shot interpreter core
+// for gfx950 (CDNA4, 64-wide wavefronts).
+//
+// Execution model ("team interpreter"):
+//   * One shot is run by one team: either a single wavefront (several teams
+//     share a workgroup, no workgroup barriers) or a whole workgroup of up to
+//     16 wavefronts.  Every lane walks the same bytecode, so opcode dispatch,
+//     the Pauli frame (p_x, p_z), gamma, the active rank k and the
+//     xoshiro256++ state are wavefront-uniform values (scalar registers for
+//     the integer parts).
+//   * The 2^k complex128 amplitudes live in LDS (k <= 13) or in a per-team
+//     slice of a global scratch buffer (larger k); array opcodes sweep them
+//     with the team's lanes striding over pair/quad indices.
+//   * Measurement probabilities and EXP_VAL sums are team reductions with a
+//     fixed association order (lane butterfly, then waves in index order), so
+//     a given launch shape is bit-reproducible.
+//   * Record bytes (measurement/detector/observable) live in LDS, one private
+//     copy per wavefront, so read-modify-write opcodes need no barrier.
+//
+// Semantics restate the reference VM opcode handlers
+// (src/clifft/svm/svm_kernels.inl:67-2173, svm_forced_kernels.cc:23-306,
+// svm.h:40-90,167-248, svm_internal.h:46-79); each handler cites its source.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "clifft_svm.h"
+
+#define CSVM_INLINE __device__ __forceinline__
+
+namespace csvm {
+
+constexpr double kInvSqrt2 = 0.70710678118654752440;
+constexpr double kDustEpsilon = 1e-18;
+constexpr double kUnreachable = -1.7976931348623157e308;
+constexpr uint32_t kNoNoise = 0xFFFFFFFFu;
+
+// Complex amplitude: a native two-lane double vector (loads/stores are single
+// 16-byte memory operations in any address space).
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+// Device copy of a program (all pointers are device memory).
+struct DevProgram {
+    const clifft_instruction* bc;
+    const uint64_t* pauli_x;
+    const uint64_t* pauli_z;
+    const uint8_t* pauli_sign;
+    const uint64_t* ev_x;
+    const uint64_t* ev_z;
+    const uint8_t* ev_sign;
+    const uint32_t* site_chan_off;
+    const uint64_t* chan_x;
+    const uint64_t* chan_z;
+    const double* chan_prob;
+    const double* hazards;
+    const uint32_t* ro_meas_idx;
+    const double* ro_prob;
+    const uint32_t* det_off;
+    const uint32_t* det_tgt;
+    const uint32_t* obs_off;
+    const uint32_t* obs_tgt;
+    const clifft_u2_node* u2;
+    const clifft_u4_node* u4;
+    const uint8_t* expected_obs;
+    uint32_t n_instr;
+    uint32_t words;
+    uint32_t peak_rank;
+    uint32_t num_meas;
+    uint32_t total_meas_slots;
+    uint32_t num_det;
+    uint32_t num_obs;
+    uint32_t num_ev;
+    uint32_t num_sites;
+    uint32_t n_expected_obs;
+};
+
+// Host-visible state blob for single-shot execution from a preset state.
+struct PresetState {
+    double gamma[2];
+    uint64_t rng[4];
+    uint32_t active_k;
+    uint32_t discarded;
+    uint32_t next_noise_idx;
+    uint32_t forced_reachable;
+    double forced_log_probability;
+    uint64_t dust_clamps;
+    uint64_t px[16];
+    uint64_t pz[16];
+};
+
+enum : uint32_t { MODE_SAMPLE = 0, MODE_FORCED = 1, MODE_PRESET = 2 };
+
+struct ShotArgs {
+    uint64_t shot_begin;  // global index of the first shot of this launch
+    uint64_t shot_count;
+    uint64_t seed;
+    uint32_t mode;
+    uint32_t flags;           // CLIFFT_REC_PACKED
+    uint32_t rec_stride;      // LDS bytes of one wavefront's record block
+    uint32_t amp_elems;       // amplitudes reserved per team (2^peak_rank)
+    uint8_t* out_meas;
+    uint8_t* out_det;
+    uint8_t* out_obs;
+    double* out_ev;
+    uint8_t* out_disc;
+    double* out_logp;                 // MODE_FORCED
+    const uint8_t* forced_records;    // MODE_FORCED [shot_count x num_meas]
+    dv2* gscratch;                    // global-resident amplitudes
+    PresetState* preset;              // MODE_PRESET
+    dv2* preset_amps;                 // MODE_PRESET amplitudes (2^peak_rank)
+    uint8_t* preset_records;          // MODE_PRESET meas|det|obs bytes
+    double* preset_ev;                // MODE_PRESET exp_vals
+    unsigned long long* dust_counter; // optional telemetry
+};
+
+// Read-only program tables are dereferenced through constant-address-space
+// pointers so uniform reads become scalar (s_load) instructions.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <class T>
+CSVM_INLINE cptr<T> cp(const T* p) {
+    return (cptr<T>)(p);
+}
+
+// Hide a table pointer's provenance from loop-invariant code motion so table
+// fields are loaded where they are used instead of pinning registers across
+// the interpreter loop.
+CSVM_INLINE uint64_t uni64_(uint64_t x) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+template <class T>
+CSVM_INLINE cptr<T> opaque(cptr<T> p) {
+    uint64_t v = uni64_((uint64_t)p);
+    __asm__ volatile("" : "+s"(v));
+    return (cptr<T>)v;
+}
+
+// ---------------------------------------------------------------------------
+// Uniform helpers
+// ---------------------------------------------------------------------------
+CSVM_INLINE uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+CSVM_INLINE uint64_t uni64(uint64_t x) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+CSVM_INLINE double unid(double x) { return __longlong_as_double((long long)uni64((uint64_t)__double_as_longlong(x))); }
+
+
+CSVM_INLINE dv2 mk(double re, double im) {
+    dv2 r;
+    r.x = re;
+    r.y = im;
+    return r;
+}
+CSVM_INLINE dv2 cmul(dv2 a, dv2 b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+CSVM_INLINE dv2 cadd(dv2 a, dv2 b) { return a + b; }
+CSVM_INLINE dv2 csub(dv2 a, dv2 b) { return a - b; }
+CSVM_INLINE dv2 cscale(dv2 a, double s) { return a * s; }
+CSVM_INLINE dv2 cneg(dv2 a) { return -a; }
+
+// svm_math.h:58-61
+CSVM_INLINE uint32_t insert_zero_bit(uint32_t val, uint32_t pos) {
+    uint32_t mask = (1u << pos) - 1u;
+    return (val & mask) | ((val & ~mask) << 1);
+}
+CSVM_INLINE uint32_t insert_two_zero_bits(uint32_t val, uint32_t lo, uint32_t hi) {
+    return insert_zero_bit(insert_zero_bit(val, lo), hi);
+}
+
+// ---------------------------------------------------------------------------
+// PRNG: xoshiro256++ seeded by SplitMix64 (svm.h:40-90), uniform per team.
+// ---------------------------------------------------------------------------
+struct Rng {
+    uint64_t s0, s1, s2, s3;
+};
+
+CSVM_INLINE uint64_t splitmix64_next(uint64_t& z) {
+    z += 0x9e3779b97f4a7c15ULL;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+// Shot s takes SplitMix64 outputs 4s..4s+3 of the stream seeded by `seed`.
+CSVM_INLINE Rng rng_for_shot(uint64_t seed, uint64_t shot) {
+    uint64_t z = seed + shot * 4ULL * 0x9e3779b97f4a7c15ULL;
+    Rng r;
+    r.s0 = splitmix64_next(z);
+    r.s1 = splitmix64_next(z);
+    r.s2 = splitmix64_next(z);
+    r.s3 = splitmix64_next(z);
+    return r;
+}
+
+CSVM_INLINE uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+CSVM_INLINE uint64_t rng_next(Rng& r) {
+    const uint64_t result = rotl64(r.s0 + r.s3, 23) + r.s0;
+    const uint64_t t = r.s1 << 17;
+    r.s2 ^= r.s0;
+    r.s3 ^= r.s1;
+    r.s1 ^= r.s2;
+    r.s0 ^= r.s3;
+    r.s2 ^= t;
+    r.s3 = rotl64(r.s3, 45);
+    return result;
+}
+
+// svm.h:150
+CSVM_INLINE double rng_double(Rng& r) { return (double)(rng_next(r) >> 11) * 0x1.0p-53; }
+
+}  // namespace csvm

@@ -1,0 +1,833 @@
+// svm_host.cpp -- C-ABI implementation: program validation and upload,
+// launch planning, and the sampling entry points (include/clifft_svm.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "svm_internal.h"
+
+using namespace csvm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess) return fail(CLIFFT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+struct DeviceInfo {
+    int ok = 0;
+    int device = 0;
+    int num_cus = 0;
+    size_t lds_max = 0;
+    std::string arch;
+};
+
+DeviceInfo& device_info() {
+    static DeviceInfo info;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return;
+        info.device = dev;
+        info.num_cus = prop.multiProcessorCount;
+        info.lds_max = 160 * 1024;
+        info.arch = prop.gcnArchName;
+        info.ok = info.arch.find("gfx950") != std::string::npos ? 1 : -1;
+    });
+    return info;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------
+// Static validation: walk the bytecode with the static active-rank
+// trajectory and check every operand the kernels index with.
+// ---------------------------------------------------------------------------
+int validate(const clifft_program_view* v, std::vector<int32_t>& khist, bool& has_forced) {
+    if (!v) return fail(CLIFFT_E_INVALID, "null program view");
+    if (v->abi_version != CLIFFT_SVM_ABI_VERSION) return fail(CLIFFT_E_INVALID, "ABI version mismatch");
+    if (v->peak_rank >= 63) return fail(CLIFFT_E_INVALID, "peak_rank >= 63 would cause undefined behavior");
+    const uint32_t nq = v->num_qubits > 0 ? v->num_qubits : std::max<uint32_t>(v->peak_rank, 1u);
+    const uint32_t words = (nq + 63) / 64;
+    if (v->words != words) return fail(CLIFFT_E_INVALID, "words != ceil(num_qubits / 64)");
+    if (words > 16) return fail(CLIFFT_E_UNSUPPORTED, "device executor supports num_qubits <= 1024");
+    if (v->total_meas_slots < v->num_measurements) return fail(CLIFFT_E_INVALID, "total_meas_slots < num_measurements");
+    if (v->num_instructions > 0xFFFFFFFFull) return fail(CLIFFT_E_UNSUPPORTED, "too many instructions");
+    if (v->num_sites > 0 && (!v->site_chan_off || !v->hazards)) return fail(CLIFFT_E_INVALID, "noise sites without CSR/hazards");
+    if (v->num_sites > 0 && v->site_chan_off[v->num_sites] > v->num_channels)
+        return fail(CLIFFT_E_INVALID, "noise channel CSR out of range");
+    // The gap sampler's upper_bound needs a finite, non-decreasing hazard
+    // table; otherwise a NOISE_BLOCK could revisit a site forever.
+    for (uint32_t s = 0; s < v->num_sites; ++s) {
+        double h = v->hazards[s];
+        if (!(h >= 0.0) || !std::isfinite(h) || (s > 0 && h < v->hazards[s - 1]))
+            return fail(CLIFFT_E_INVALID, "noise hazard table must be finite and non-decreasing");
+    }
+    for (uint32_t i = 0; i < v->num_readout; ++i)
+        if (v->ro_meas_idx[i] >= v->total_meas_slots) return fail(CLIFFT_E_INVALID, "readout meas_idx out of range");
+    if (v->num_det_lists > 0 && v->det_off[v->num_det_lists] > 0)
+        for (uint32_t j = 0; j < v->det_off[v->num_det_lists]; ++j)
+            if (v->det_tgt[j] >= v->total_meas_slots) return fail(CLIFFT_E_INVALID, "detector target out of range");
+    if (v->num_obs_lists > 0 && v->obs_off[v->num_obs_lists] > 0)
+        for (uint32_t j = 0; j < v->obs_off[v->num_obs_lists]; ++j)
+            if (v->obs_tgt[j] >= v->total_meas_slots) return fail(CLIFFT_E_INVALID, "observable target out of range");
+
+    const uint32_t fbits = words * 64;
+    has_forced = false;
+    khist.assign(v->num_instructions, 0);
+    int32_t k = 0, kmax = 0;
+    auto bad = [&](uint64_t pc, const char* what) {
+        return fail(CLIFFT_E_INVALID, "instruction " + std::to_string(pc) + ": " + what);
+    };
+    for (uint64_t pc = 0; pc < v->num_instructions; ++pc) {
+        const clifft_instruction& in = v->bytecode[pc];
+        khist[pc] = k;
+        if (in.opcode >= CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED && in.opcode <= CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED)
+            has_forced = true;
+        const uint32_t a1 = in.axis_1, a2 = in.axis_2, i0 = in.p.pauli.cp_mask_idx, i1 = in.p.pauli.condition_idx;
+        switch (in.opcode) {
+            case CLIFFT_OP_FRAME_CNOT:
+            case CLIFFT_OP_FRAME_CZ:
+            case CLIFFT_OP_FRAME_SWAP:
+                if (a1 >= fbits || a2 >= fbits) return bad(pc, "frame axis out of range");
+                break;
+            case CLIFFT_OP_FRAME_H:
+            case CLIFFT_OP_FRAME_S:
+            case CLIFFT_OP_FRAME_S_DAG:
+                if (a1 >= fbits) return bad(pc, "frame axis out of range");
+                break;
+            case CLIFFT_OP_ARRAY_CNOT:
+            case CLIFFT_OP_ARRAY_CZ:
+            case CLIFFT_OP_ARRAY_SWAP:
+                if (a1 >= (uint32_t)k || a2 >= (uint32_t)k || a1 == a2) return bad(pc, "2-axis array op needs distinct active axes");
+                break;
+            case CLIFFT_OP_ARRAY_MULTI_CNOT:
+            case CLIFFT_OP_ARRAY_MULTI_CZ:
+                if (a1 >= (uint32_t)k) return bad(pc, "multi-gate axis not active");
+                if (in.p.multi_gate.mask & (1ULL << a1)) return bad(pc, "multi-gate mask contains its own axis");
+                break;
+            case CLIFFT_OP_ARRAY_H:
+            case CLIFFT_OP_ARRAY_S:
+            case CLIFFT_OP_ARRAY_S_DAG:
+                if (a1 >= (uint32_t)k) return bad(pc, "array axis not active");
+                break;
+            case CLIFFT_OP_ARRAY_T:
+            case CLIFFT_OP_ARRAY_T_DAG:
+            case CLIFFT_OP_ARRAY_ROT:
+                if (a1 >= fbits) return bad(pc, "axis out of range");
+                break;
+            case CLIFFT_OP_ARRAY_U2:
+                if (a1 >= fbits) return bad(pc, "axis out of range");
+                if (in.p.u.cp_idx >= v->num_u2) return bad(pc, "U2 node index out of range");
+                break;
+            case CLIFFT_OP_ARRAY_U4:
+                if (!(a1 < a2 && a2 < (uint32_t)k)) return bad(pc, "U4 needs lo < hi < k");
+                if (in.p.u.cp_idx >= v->num_u4) return bad(pc, "U4 node index out of range");
+                break;
+            case CLIFFT_OP_EXPAND:
+            case CLIFFT_OP_EXPAND_T:
+            case CLIFFT_OP_EXPAND_T_DAG:
+            case CLIFFT_OP_EXPAND_ROT:
+                if (a1 != (uint32_t)k) return bad(pc, "EXPAND must target axis k");
+                ++k;
+                if ((uint32_t)k > v->peak_rank) return bad(pc, "EXPAND exceeds peak_rank");
+                break;
+            case CLIFFT_OP_MEAS_DORMANT_STATIC:
+            case CLIFFT_OP_MEAS_DORMANT_RANDOM:
+            case CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED:
+            case CLIFFT_OP_MEAS_DORMANT_RANDOM_FORCED:
+                if (a1 >= fbits) return bad(pc, "axis out of range");
+                if (in.p.classical.classical_idx >= v->total_meas_slots) return bad(pc, "record index out of range");
+                break;
+            case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL:
+            case CLIFFT_OP_MEAS_ACTIVE_INTERFERE:
+            case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL_FORCED:
+            case CLIFFT_OP_MEAS_ACTIVE_INTERFERE_FORCED:
+                if (k < 1 || a1 != (uint32_t)(k - 1)) return bad(pc, "active measurement must target axis k-1");
+                if (in.p.classical.classical_idx >= v->total_meas_slots) return bad(pc, "record index out of range");
+                --k;
+                break;
+            case CLIFFT_OP_SWAP_MEAS_INTERFERE:
+            case CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED:
+                if (k < 1 || a2 != (uint32_t)(k - 1) || a1 >= (uint32_t)k) return bad(pc, "swap-measure axes invalid");
+                if (in.p.classical.classical_idx >= v->total_meas_slots) return bad(pc, "record index out of range");
+                --k;
+                break;
+            case CLIFFT_OP_APPLY_PAULI:
+                if (i0 >= v->num_pauli_masks || i1 >= v->total_meas_slots) return bad(pc, "APPLY_PAULI index out of range");
+                break;
+            case CLIFFT_OP_NOISE:
+                if (i0 >= v->num_sites) return bad(pc, "noise site out of range");
+                break;
+            case CLIFFT_OP_NOISE_BLOCK:
+                if ((uint64_t)i0 + i1 > v->num_sites) return bad(pc, "noise block out of range");
+                break;
+            case CLIFFT_OP_READOUT_NOISE:
+                if (i0 >= v->num_readout) return bad(pc, "readout entry out of range");
+                break;
+            case CLIFFT_OP_DETECTOR:
+            case CLIFFT_OP_POSTSELECT:
+                if (i0 >= v->num_det_lists || i1 >= v->num_detectors) return bad(pc, "detector index out of range");
+                break;
+            case CLIFFT_OP_OBSERVABLE:
+                if (i0 >= v->num_obs_lists || i1 >= v->num_observables) return bad(pc, "observable index out of range");
+                break;
+            case CLIFFT_OP_EXP_VAL:
+                if (in.p.exp_val.cp_exp_val_idx >= v->num_ev_masks || in.p.exp_val.exp_val_idx >= v->num_exp_vals)
+                    return bad(pc, "EXP_VAL index out of range");
+                break;
+            default:
+                return bad(pc, "unknown opcode");
+        }
+        kmax = std::max(kmax, k);
+    }
+    if ((uint32_t)kmax > v->peak_rank) return fail(CLIFFT_E_INVALID, "active rank exceeds peak_rank");
+    return CLIFFT_OK;
+}
+
+// Algorithmic bytes of one array instruction at pre-op rank k (16 B per
+// amplitude; read + write of what the opcode must touch).
+double op_bytes(const clifft_instruction& in, int32_t k, uint32_t fbits) {
+    double K = std::ldexp(1.0, k);
+    switch (in.opcode) {
+        case CLIFFT_OP_ARRAY_H:
+        case CLIFFT_OP_ARRAY_U4:
+        case CLIFFT_OP_EXPAND:
+        case CLIFFT_OP_EXPAND_T:
+        case CLIFFT_OP_EXPAND_T_DAG:
+        case CLIFFT_OP_EXPAND_ROT:
+            return 32.0 * K;
+        case CLIFFT_OP_ARRAY_U2:
+            return in.axis_1 < (uint32_t)k ? 32.0 * K : 0.0;
+        case CLIFFT_OP_ARRAY_CNOT:
+        case CLIFFT_OP_ARRAY_SWAP:
+        case CLIFFT_OP_ARRAY_MULTI_CNOT:
+        case CLIFFT_OP_ARRAY_S:
+        case CLIFFT_OP_ARRAY_S_DAG:
+            return 16.0 * K;
+        case CLIFFT_OP_ARRAY_T:
+        case CLIFFT_OP_ARRAY_T_DAG:
+        case CLIFFT_OP_ARRAY_ROT:
+            return in.axis_1 < (uint32_t)k ? 16.0 * K : 0.0;
+        case CLIFFT_OP_ARRAY_CZ:
+        case CLIFFT_OP_ARRAY_MULTI_CZ:
+            return 8.0 * K;
+        case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL:
+        case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL_FORCED:
+            return 16.0 * K;
+        case CLIFFT_OP_MEAS_ACTIVE_INTERFERE:
+        case CLIFFT_OP_SWAP_MEAS_INTERFERE:
+        case CLIFFT_OP_MEAS_ACTIVE_INTERFERE_FORCED:
+        case CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED:
+            return 40.0 * K;
+        case CLIFFT_OP_EXP_VAL:
+            return 16.0 * K;
+        default:
+            (void)fbits;
+            return 0.0;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Program object
+// ---------------------------------------------------------------------------
+struct clifft_svm_program {
+    hipStream_t stream = nullptr;
+    void* dev_blob = nullptr;
+    DevProgram dp{};                          // host copy (device pointers)
+    DevProgram* dp_dev = nullptr;             // device copy, sampling bytecode
+    DevProgram* dp_forced_dev = nullptr;      // device copy, *_FORCED bytecode
+    clifft_instruction* forced_bc = nullptr;  // device bytecode with *_FORCED opcodes
+    std::vector<clifft_instruction> host_bc;
+    std::vector<int32_t> khist;
+    std::vector<uint8_t> expected_obs;
+    uint32_t num_qubits = 0, peak_rank = 0, words = 1, nw = 1;
+    uint32_t num_meas = 0, total_slots = 0, num_det = 0, num_obs = 0, num_ev = 0;
+    bool has_postselection = false;
+    bool has_forced_ops = false;  // bytecode already contains *_FORCED opcodes
+    uint64_t num_array_ops = 0;
+    double bytes_per_shot = 0.0;
+    // launch planning
+    LaunchShape base_shape;
+    // scratch (grown on demand)
+    void* gscratch = nullptr;
+    size_t gscratch_bytes = 0;
+    void* out_dev = nullptr;
+    size_t out_dev_bytes = 0;
+    unsigned long long* dust_dev = nullptr;
+    ShotArgs* args_dev = nullptr;  // ring of launch-argument slots
+    uint32_t args_slot = 0;
+};
+
+static constexpr uint32_t kArgSlots = 64;
+
+namespace {
+
+int frame_nw(uint32_t words) {
+    return words <= 1 ? 1 : 0;  // 0 = LDS-resident frame
+}
+
+size_t rec_stride_of(const clifft_svm_program* p) {
+    size_t r = align_up(std::max<size_t>(p->total_slots + p->num_det + p->num_obs, 4), 16);
+    return r + (p->nw == 0 ? 256 : 0);
+}
+
+// Choose the team shape and storage tier for a program's peak rank.
+LaunchShape plan_shape(const clifft_svm_program* p) {
+    LaunchShape L;
+    L.nw = (int)p->nw;
+    const uint32_t k = p->peak_rank;
+    const size_t rec_stride = rec_stride_of(p);
+    const size_t amp_bytes = (size_t)16 << k;
+    const size_t red = 16 * 4 * sizeof(double);
+    if (k <= 7) {
+        L.block_team = false;
+        L.block = 256;
+        L.global_amps = false;
+    } else if (k <= 13) {
+        L.block_team = true;
+        uint32_t waves = k <= 9 ? 2 : (k <= 10 ? 4 : (k <= 12 ? 8 : 16));
+        L.block = 64 * waves;
+        L.global_amps = false;
+    } else {
+        L.block_team = true;
+        L.block = 1024;
+        L.global_amps = true;
+    }
+    const uint32_t waves = L.block / 64;
+    size_t lds = red + waves * rec_stride;
+    if (!L.global_amps) lds += L.teams_per_block() * amp_bytes;
+    if (lds > 160 * 1024) {  // records too large next to LDS amplitudes
+        L.global_amps = true;
+        lds = red + waves * rec_stride;
+    }
+    L.lds_bytes = lds;
+    return L;
+}
+
+int ensure_scratch(clifft_svm_program* p, size_t bytes) {
+    if (bytes <= p->gscratch_bytes) return CLIFFT_OK;
+    if (p->gscratch) (void)hipFree(p->gscratch);
+    p->gscratch = nullptr;
+    p->gscratch_bytes = 0;
+    HIP_TRY(hipMalloc(&p->gscratch, bytes));
+    p->gscratch_bytes = bytes;
+    return CLIFFT_OK;
+}
+
+int ensure_out(clifft_svm_program* p, size_t bytes) {
+    if (bytes <= p->out_dev_bytes) return CLIFFT_OK;
+    if (p->out_dev) (void)hipFree(p->out_dev);
+    p->out_dev = nullptr;
+    p->out_dev_bytes = 0;
+    HIP_TRY(hipMalloc(&p->out_dev, bytes));
+    p->out_dev_bytes = bytes;
+    return CLIFFT_OK;
+}
+
+// Size the grid: persistent teams up to the resident capacity of the chip.
+int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots) {
+    DeviceInfo& di = device_info();
+    const uint32_t tpb = L.teams_per_block();
+    uint64_t blocks_needed = (shots + tpb - 1) / tpb;
+    uint32_t waves = L.block / 64;
+    uint32_t by_lds = L.lds_bytes > 0 ? (uint32_t)std::max<size_t>(1, (160 * 1024) / L.lds_bytes) : 32;
+    uint32_t by_waves = std::max<uint32_t>(1, 32 / waves);
+    uint32_t per_cu = std::min(by_lds, by_waves);
+    uint64_t cap = (uint64_t)di.num_cus * per_cu;
+    if (L.global_amps) {
+        // bound the global scratch to 8 GiB
+        size_t per_team = (size_t)16 << p->peak_rank;
+        uint64_t max_teams = std::max<uint64_t>(1, (8ull << 30) / per_team);
+        cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, max_teams / tpb));
+    }
+    L.grid = (uint32_t)std::max<uint64_t>(1, std::min(blocks_needed, cap));
+    if (L.global_amps) {
+        size_t need = (size_t)L.grid * tpb * ((size_t)16 << p->peak_rank);
+        int rc = ensure_scratch(p, need);
+        if (rc) return rc;
+    }
+    return CLIFFT_OK;
+}
+
+void fill_common_args(clifft_svm_program* p, ShotArgs& A) {
+    std::memset(&A, 0, sizeof(A));
+    A.rec_stride = (uint32_t)rec_stride_of(p);
+    A.amp_elems = 1u << p->peak_rank;
+    A.gscratch = reinterpret_cast<dv2*>(p->gscratch);
+    A.dust_counter = p->dust_dev;
+}
+
+size_t row_bytes(uint32_t n, bool packed) { return packed ? (n + 7) / 8 : n; }
+
+// Copy the launch arguments into the next device slot and launch.  The
+// host copy is staged by the runtime before hipMemcpyAsync returns, so the
+// caller's ShotArgs may be reused immediately.
+hipError_t launch_with_args(clifft_svm_program* p, const LaunchShape& L, const DevProgram* dp, const ShotArgs& A,
+                            hipStream_t st) {
+    ShotArgs* slot = p->args_dev + (p->args_slot++ % kArgSlots);
+    hipError_t e = hipMemcpyAsync(slot, &A, sizeof(ShotArgs), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    return launch_shots(L, dp, slot, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t clifft_svm_abi_version(void) { return CLIFFT_SVM_ABI_VERSION; }
+
+const char* clifft_svm_backend(void) {
+    DeviceInfo& di = device_info();
+    if (di.ok == 1) return "hip:gfx950";
+    if (di.ok == -1) return "hip:unsupported-device";
+    return "hip:no-device";
+}
+
+const char* clifft_svm_last_error(void) { return g_last_error.c_str(); }
+
+int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_program** out) {
+    if (!out) return fail(CLIFFT_E_INVALID, "null output handle");
+    *out = nullptr;
+    std::vector<int32_t> khist;
+    bool has_forced = false;
+    int rc = validate(view, khist, has_forced);
+    if (rc) return rc;
+    DeviceInfo& di = device_info();
+    if (di.ok == 0) return fail(CLIFFT_E_NODEVICE, "no HIP device visible");
+    if (di.ok < 0) return fail(CLIFFT_E_NODEVICE, "device is " + di.arch + ", this build targets gfx950");
+
+    auto* p = new clifft_svm_program();
+    p->num_qubits = view->num_qubits;
+    p->peak_rank = view->peak_rank;
+    p->words = view->words;
+    p->nw = frame_nw(view->words);
+    p->num_meas = view->num_measurements;
+    p->total_slots = view->total_meas_slots;
+    p->num_det = view->num_detectors;
+    p->num_obs = view->num_observables;
+    p->num_ev = view->num_exp_vals;
+    p->has_postselection = view->has_postselection != 0;
+    p->has_forced_ops = has_forced;
+    p->host_bc.assign(view->bytecode, view->bytecode + view->num_instructions);
+    p->khist = khist;
+    if (view->expected_observables)
+        p->expected_obs.assign(view->expected_observables, view->expected_observables + view->num_expected_observables);
+    const uint32_t fbits = view->words * 64;
+    for (uint64_t i = 0; i < view->num_instructions; ++i) {
+        double b = op_bytes(p->host_bc[i], khist[i], fbits);
+        if (b > 0) {
+            p->num_array_ops++;
+            p->bytes_per_shot += b;
+        }
+    }
+
+    // Pack every array into one device allocation (256-B aligned pieces).
+    struct Piece {
+        const void* src;
+        size_t bytes;
+        size_t off;
+    };
+    std::vector<Piece> pieces;
+    size_t total = 0;
+    auto add = [&](const void* src, size_t bytes) -> size_t {
+        size_t off = total;
+        pieces.push_back({src, bytes, off});
+        total = align_up(total + std::max<size_t>(bytes, 8), 256);
+        return off;
+    };
+    const size_t W = view->words;
+    size_t o_bc = add(view->bytecode, view->num_instructions * sizeof(clifft_instruction));
+    size_t o_fbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
+    size_t o_px = add(view->pauli_x, view->num_pauli_masks * W * 8);
+    size_t o_pz = add(view->pauli_z, view->num_pauli_masks * W * 8);
+    size_t o_ps = add(view->pauli_sign, view->num_pauli_masks);
+    size_t o_ex = add(view->ev_x, view->num_ev_masks * W * 8);
+    size_t o_ez = add(view->ev_z, view->num_ev_masks * W * 8);
+    size_t o_es = add(view->ev_sign, view->num_ev_masks);
+    size_t o_so = add(view->site_chan_off, view->num_sites > 0 ? (view->num_sites + 1) * 4 : 0);
+    size_t o_cx = add(view->chan_x, view->num_channels * W * 8);
+    size_t o_cz = add(view->chan_z, view->num_channels * W * 8);
+    size_t o_cp = add(view->chan_prob, view->num_channels * 8);
+    size_t o_hz = add(view->hazards, view->num_sites * 8);
+    size_t o_rm = add(view->ro_meas_idx, view->num_readout * 4);
+    size_t o_rp = add(view->ro_prob, view->num_readout * 8);
+    size_t o_do = add(view->det_off, view->num_det_lists > 0 ? (view->num_det_lists + 1) * 4 : 0);
+    size_t o_dt = add(view->det_tgt, view->num_det_lists > 0 ? view->det_off[view->num_det_lists] * 4 : 0);
+    size_t o_oo = add(view->obs_off, view->num_obs_lists > 0 ? (view->num_obs_lists + 1) * 4 : 0);
+    size_t o_ot = add(view->obs_tgt, view->num_obs_lists > 0 ? view->obs_off[view->num_obs_lists] * 4 : 0);
+    size_t o_u2 = add(view->u2_nodes, view->num_u2 * sizeof(clifft_u2_node));
+    size_t o_u4 = add(view->u4_nodes, view->num_u4 * sizeof(clifft_u4_node));
+    size_t o_eo = add(view->expected_observables, view->num_expected_observables);
+    size_t o_dp = add(nullptr, 2 * sizeof(DevProgram));
+
+    std::vector<uint8_t> staging(total, 0);
+    for (const Piece& pc : pieces)
+        if (pc.src && pc.bytes) std::memcpy(staging.data() + pc.off, pc.src, pc.bytes);
+    // forced copy of the bytecode (api/record_probabilities.cc:96-118)
+    {
+        auto* fbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_fbc);
+        std::memcpy(fbc, view->bytecode, view->num_instructions * sizeof(clifft_instruction));
+        for (uint64_t i = 0; i < view->num_instructions; ++i) {
+            uint8_t& op = fbc[i].opcode;
+            if (op >= CLIFFT_OP_MEAS_DORMANT_STATIC && op <= CLIFFT_OP_SWAP_MEAS_INTERFERE)
+                op = (uint8_t)(op + (CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED - CLIFFT_OP_MEAS_DORMANT_STATIC));
+        }
+    }
+    hipError_t e = hipMalloc(&p->dev_blob, total);
+    if (e == hipSuccess) e = hipMemcpy(p->dev_blob, staging.data(), total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreate(&p->stream);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->dust_dev), sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(p->dust_dev, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->args_dev), kArgSlots * sizeof(ShotArgs));
+    if (e != hipSuccess) {
+        std::string msg = std::string("program upload: ") + hipGetErrorString(e);
+        clifft_svm_program_destroy(p);
+        return fail(CLIFFT_E_HIP, msg);
+    }
+    auto* base = static_cast<uint8_t*>(p->dev_blob);
+    DevProgram& d = p->dp;
+    d.bc = reinterpret_cast<const clifft_instruction*>(base + o_bc);
+    p->forced_bc = reinterpret_cast<clifft_instruction*>(base + o_fbc);
+    d.pauli_x = reinterpret_cast<const uint64_t*>(base + o_px);
+    d.pauli_z = reinterpret_cast<const uint64_t*>(base + o_pz);
+    d.pauli_sign = base + o_ps;
+    d.ev_x = reinterpret_cast<const uint64_t*>(base + o_ex);
+    d.ev_z = reinterpret_cast<const uint64_t*>(base + o_ez);
+    d.ev_sign = base + o_es;
+    d.site_chan_off = reinterpret_cast<const uint32_t*>(base + o_so);
+    d.chan_x = reinterpret_cast<const uint64_t*>(base + o_cx);
+    d.chan_z = reinterpret_cast<const uint64_t*>(base + o_cz);
+    d.chan_prob = reinterpret_cast<const double*>(base + o_cp);
+    d.hazards = reinterpret_cast<const double*>(base + o_hz);
+    d.ro_meas_idx = reinterpret_cast<const uint32_t*>(base + o_rm);
+    d.ro_prob = reinterpret_cast<const double*>(base + o_rp);
+    d.det_off = reinterpret_cast<const uint32_t*>(base + o_do);
+    d.det_tgt = reinterpret_cast<const uint32_t*>(base + o_dt);
+    d.obs_off = reinterpret_cast<const uint32_t*>(base + o_oo);
+    d.obs_tgt = reinterpret_cast<const uint32_t*>(base + o_ot);
+    d.u2 = reinterpret_cast<const clifft_u2_node*>(base + o_u2);
+    d.u4 = reinterpret_cast<const clifft_u4_node*>(base + o_u4);
+    d.expected_obs = base + o_eo;
+    d.n_instr = (uint32_t)view->num_instructions;
+    d.words = view->words;
+    d.peak_rank = view->peak_rank;
+    d.num_meas = view->num_measurements;
+    d.total_meas_slots = view->total_meas_slots;
+    d.num_det = view->num_detectors;
+    d.num_obs = view->num_observables;
+    d.num_ev = view->num_exp_vals;
+    d.num_sites = view->num_sites;
+    d.n_expected_obs = view->num_expected_observables;
+    {
+        DevProgram both[2] = {d, d};
+        both[1].bc = p->forced_bc;
+        e = hipMemcpy(base + o_dp, both, sizeof(both), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            std::string msg = std::string("program upload: ") + hipGetErrorString(e);
+            clifft_svm_program_destroy(p);
+            return fail(CLIFFT_E_HIP, msg);
+        }
+        p->dp_dev = reinterpret_cast<DevProgram*>(base + o_dp);
+        p->dp_forced_dev = p->dp_dev + 1;
+    }
+    p->base_shape = plan_shape(p);
+    *out = p;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_program_destroy(clifft_svm_program* p) {
+    if (!p) return CLIFFT_OK;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->dev_blob) (void)hipFree(p->dev_blob);
+    if (p->gscratch) (void)hipFree(p->gscratch);
+    if (p->out_dev) (void)hipFree(p->out_dev);
+    if (p->dust_dev) (void)hipFree(p->dust_dev);
+    if (p->args_dev) (void)hipFree(p->args_dev);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_program_info_get(const clifft_svm_program* p, clifft_svm_program_info* info) {
+    if (!p || !info) return fail(CLIFFT_E_INVALID, "null argument");
+    info->peak_rank = p->peak_rank;
+    info->tier = p->base_shape.global_amps ? 1u : 0u;
+    info->team_waves = p->base_shape.block_team ? p->base_shape.block / 64 : 1;
+    info->shots_per_block = p->base_shape.teams_per_block();
+    info->num_array_ops = p->num_array_ops;
+    info->bytes_per_shot = p->bytes_per_shot;
+    return CLIFFT_OK;
+}
+
+// Shared implementation of sample / forced runs.  Outputs go to device
+// memory (either the caller's, or the program's staging buffer which is then
+// copied to the caller's host arrays).
+static int run_shots(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags, uint32_t mode,
+                     const clifft_svm_records* out, const uint8_t* forced_host, double* logp_host,
+                     hipStream_t st) {
+    const bool dev_out = (flags & CLIFFT_REC_DEVICE) != 0;
+    const bool packed = (flags & CLIFFT_REC_PACKED) != 0;
+    if (mode == MODE_SAMPLE && p->has_forced_ops)
+        return fail(CLIFFT_E_INVALID, "program contains *_FORCED opcodes; use clifft_svm_record_log_probs");
+    const uint64_t chunk_max = dev_out ? shots : (uint64_t)1 << 20;
+    for (uint64_t base = 0; base < shots; base += chunk_max) {
+        const uint64_t n = std::min(chunk_max, shots - base);
+        LaunchShape L = p->base_shape;
+        int rc = finalize_shape(p, L, n);
+        if (rc) return rc;
+        ShotArgs A;
+        fill_common_args(p, A);
+        A.shot_begin = base;
+        A.shot_count = n;
+        A.seed = seed;
+        A.mode = mode;
+        A.flags = flags;
+        size_t bm = row_bytes(p->num_meas, packed), bd = row_bytes(p->num_det, packed), bo = row_bytes(p->num_obs, packed);
+        if (dev_out) {
+            if (out) {
+                A.out_meas = out->measurements ? out->measurements + base * bm : nullptr;
+                A.out_det = out->detectors ? out->detectors + base * bd : nullptr;
+                A.out_obs = out->observables ? out->observables + base * bo : nullptr;
+                A.out_ev = out->exp_vals ? out->exp_vals + base * p->num_ev : nullptr;
+                A.out_disc = out->discarded ? out->discarded + base : nullptr;
+            }
+        } else {
+            // device staging: [meas | det | obs | disc | ev | logp | forced]
+            size_t s_m = align_up(n * bm, 256), s_d = align_up(n * bd, 256), s_o = align_up(n * bo, 256);
+            size_t s_x = align_up(n, 256), s_e = align_up(n * p->num_ev * 8, 256), s_l = align_up(n * 8, 256);
+            size_t s_f = forced_host ? align_up(n * p->num_meas, 256) : 0;
+            rc = ensure_out(p, s_m + s_d + s_o + s_x + s_e + s_l + s_f + 256);
+            if (rc) return rc;
+            uint8_t* o = static_cast<uint8_t*>(p->out_dev);
+            A.out_meas = (out && out->measurements) ? o : nullptr;
+            A.out_det = (out && out->detectors) ? o + s_m : nullptr;
+            A.out_obs = (out && out->observables) ? o + s_m + s_d : nullptr;
+            A.out_disc = (out && out->discarded) ? o + s_m + s_d + s_o : nullptr;
+            A.out_ev = (out && out->exp_vals) ? reinterpret_cast<double*>(o + s_m + s_d + s_o + s_x) : nullptr;
+            A.out_logp = reinterpret_cast<double*>(o + s_m + s_d + s_o + s_x + s_e);
+            if (forced_host) {
+                uint8_t* fdev = o + s_m + s_d + s_o + s_x + s_e + s_l;
+                HIP_TRY(hipMemcpyAsync(fdev, forced_host + base * p->num_meas, n * p->num_meas, hipMemcpyHostToDevice, st));
+                A.forced_records = fdev;
+            }
+        }
+        HIP_TRY(launch_with_args(p, L, mode == MODE_FORCED ? p->dp_forced_dev : p->dp_dev, A, st));
+        if (!dev_out) {
+            if (out && out->measurements && n * bm)
+                HIP_TRY(hipMemcpyAsync(out->measurements + base * bm, A.out_meas, n * bm, hipMemcpyDeviceToHost, st));
+            if (out && out->detectors && n * bd)
+                HIP_TRY(hipMemcpyAsync(out->detectors + base * bd, A.out_det, n * bd, hipMemcpyDeviceToHost, st));
+            if (out && out->observables && n * bo)
+                HIP_TRY(hipMemcpyAsync(out->observables + base * bo, A.out_obs, n * bo, hipMemcpyDeviceToHost, st));
+            if (out && out->discarded)
+                HIP_TRY(hipMemcpyAsync(out->discarded + base, A.out_disc, n, hipMemcpyDeviceToHost, st));
+            if (out && out->exp_vals && p->num_ev)
+                HIP_TRY(hipMemcpyAsync(out->exp_vals + base * p->num_ev, A.out_ev, n * p->num_ev * 8, hipMemcpyDeviceToHost, st));
+            if (logp_host)
+                HIP_TRY(hipMemcpyAsync(logp_host + base, A.out_logp, n * 8, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+    }
+    return CLIFFT_OK;
+}
+
+int clifft_svm_sample(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags,
+                      const clifft_svm_records* out, void* stream) {
+    if (!p) return fail(CLIFFT_E_INVALID, "null program");
+    if (shots == 0) return CLIFFT_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    int rc = run_shots(p, shots, seed, flags, MODE_SAMPLE, out, nullptr, nullptr, st);
+    if (rc) return rc;
+    if (flags & CLIFFT_REC_DEVICE) HIP_TRY(hipStreamSynchronize(st));
+    return CLIFFT_OK;
+}
+
+int clifft_svm_sample_survivors(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags,
+                                clifft_svm_survivors* summary, const clifft_svm_records* out, void* stream) {
+    if (!p || !summary) return fail(CLIFFT_E_INVALID, "null argument");
+    if (flags & CLIFFT_REC_DEVICE) return fail(CLIFFT_E_UNSUPPORTED, "survivor compaction writes host memory");
+    summary->total_shots = shots;
+    summary->passed_shots = 0;
+    summary->logical_errors = 0;
+    if (summary->observable_ones)
+        for (uint32_t i = 0; i < p->num_obs; ++i) summary->observable_ones[i] = 0;
+    if (shots == 0) return CLIFFT_OK;
+    const bool packed = (flags & CLIFFT_REC_PACKED) != 0;
+    std::vector<uint8_t> disc(shots), obs((size_t)shots * p->num_obs);
+    std::vector<uint8_t> meas, det;
+    std::vector<double> ev;
+    clifft_svm_records r{};
+    r.discarded = disc.data();
+    r.observables = p->num_obs ? obs.data() : nullptr;
+    if (out) {
+        meas.resize((size_t)shots * p->num_meas);
+        det.resize((size_t)shots * p->num_det);
+        ev.resize((size_t)shots * p->num_ev);
+        r.measurements = out->measurements ? meas.data() : nullptr;
+        r.detectors = out->detectors ? det.data() : nullptr;
+        r.exp_vals = out->exp_vals ? ev.data() : nullptr;
+    }
+    // observables are always fetched unpacked for counting
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    int rc = run_shots(p, shots, seed, flags & ~CLIFFT_REC_PACKED, MODE_SAMPLE, &r, nullptr, nullptr, st);
+    if (rc) return rc;
+    const size_t bm = row_bytes(p->num_meas, false), bd = row_bytes(p->num_det, false);
+    const size_t pm = row_bytes(p->num_meas, packed), pd = row_bytes(p->num_det, packed), po = row_bytes(p->num_obs, packed);
+    auto put_row = [&](uint8_t* dst, const uint8_t* src, uint32_t n) {
+        if (!packed) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        for (uint32_t j = 0; j < (n + 7) / 8; ++j) {
+            uint8_t b = 0;
+            for (uint32_t i = 8 * j; i < std::min(n, 8 * j + 8); ++i) b |= (uint8_t)((src[i] & 1u) << (i - 8 * j));
+            dst[j] = b;
+        }
+    };
+    uint64_t row = 0;
+    for (uint64_t s = 0; s < shots; ++s) {
+        if (disc[s]) continue;
+        bool any = false;
+        for (uint32_t i = 0; i < p->num_obs; ++i) {
+            uint8_t v = obs[s * p->num_obs + i];
+            if (v) {
+                any = true;
+                if (summary->observable_ones) summary->observable_ones[i]++;
+            }
+        }
+        if (any) summary->logical_errors++;
+        if (out) {
+            if (out->observables && p->num_obs) put_row(out->observables + row * po, obs.data() + s * p->num_obs, p->num_obs);
+            if (out->measurements && bm) put_row(out->measurements + row * pm, meas.data() + s * bm, p->num_meas);
+            if (out->detectors && bd) put_row(out->detectors + row * pd, det.data() + s * bd, p->num_det);
+            if (out->exp_vals && p->num_ev)
+                std::memcpy(out->exp_vals + row * p->num_ev, ev.data() + s * p->num_ev, p->num_ev * 8);
+        }
+        row++;
+    }
+    summary->passed_shots = row;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_record_log_probs(clifft_svm_program* p, const uint8_t* records, uint64_t num_records,
+                                double* log_probs, void* stream) {
+    if (!p || (!records && num_records) || (!log_probs && num_records)) return fail(CLIFFT_E_INVALID, "null argument");
+    if (p->total_slots != p->num_meas)
+        return fail(CLIFFT_E_INVALID, "record_log_probs: program has hidden measurement slots");
+    if (num_records == 0) return CLIFFT_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    return run_shots(p, num_records, 0, 0, MODE_FORCED, nullptr, records, log_probs, st);
+}
+
+int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
+    if (!p || !s || !s->v || !s->px || !s->pz) return fail(CLIFFT_E_INVALID, "null argument");
+    if (s->active_k > p->peak_rank) return fail(CLIFFT_E_INVALID, "state active_k exceeds peak_rank");
+    if (p->has_forced_ops && !s->forced_record)
+        return fail(CLIFFT_E_INVALID, "program contains *_FORCED opcodes but the state has no forced_record");
+    hipStream_t st = p->stream;
+    const size_t namp = (size_t)1 << p->peak_rank;
+    const size_t nrec = (size_t)p->total_slots + p->num_det + p->num_obs;
+    PresetState ps{};
+    ps.gamma[0] = s->gamma_re;
+    ps.gamma[1] = s->gamma_im;
+    for (int i = 0; i < 4; ++i) ps.rng[i] = s->rng[i];
+    ps.active_k = s->active_k;
+    ps.discarded = s->discarded;
+    ps.next_noise_idx = s->next_noise_idx;
+    ps.forced_reachable = s->forced_reachable;
+    ps.forced_log_probability = s->forced_log_probability;
+    ps.dust_clamps = s->dust_clamps;
+    for (uint32_t w = 0; w < p->words; ++w) {
+        ps.px[w] = s->px[w];
+        ps.pz[w] = s->pz[w];
+    }
+    std::vector<uint8_t> recs(nrec + 16, 0);
+    if (s->meas_record) std::memcpy(recs.data(), s->meas_record, p->total_slots);
+    if (s->det_record) std::memcpy(recs.data() + p->total_slots, s->det_record, p->num_det);
+    if (s->obs_record) std::memcpy(recs.data() + p->total_slots + p->num_det, s->obs_record, p->num_obs);
+    size_t s_ps = align_up(sizeof(PresetState), 256), s_a = align_up(namp * 16, 256), s_r = align_up(nrec + 16, 256);
+    size_t s_e = align_up((p->num_ev + 1) * 8, 256), s_f = align_up(p->total_slots + 16, 256);
+    int rc = ensure_out(p, s_ps + s_a + s_r + s_e + s_f);
+    if (rc) return rc;
+    uint8_t* o = static_cast<uint8_t*>(p->out_dev);
+    auto* d_ps = reinterpret_cast<PresetState*>(o);
+    auto* d_amp = reinterpret_cast<dv2*>(o + s_ps);
+    uint8_t* d_rec = o + s_ps + s_a;
+    auto* d_ev = reinterpret_cast<double*>(o + s_ps + s_a + s_r);
+    uint8_t* d_forced = o + s_ps + s_a + s_r + s_e;
+    HIP_TRY(hipMemcpyAsync(d_ps, &ps, sizeof(ps), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_amp, s->v, namp * 16, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_rec, recs.data(), nrec, hipMemcpyHostToDevice, st));
+    if (s->exp_vals && p->num_ev) HIP_TRY(hipMemcpyAsync(d_ev, s->exp_vals, p->num_ev * 8, hipMemcpyHostToDevice, st));
+    if (s->forced_record) HIP_TRY(hipMemcpyAsync(d_forced, s->forced_record, p->total_slots, hipMemcpyHostToDevice, st));
+    LaunchShape L = p->base_shape;
+    rc = finalize_shape(p, L, 1);
+    if (rc) return rc;
+    ShotArgs A;
+    fill_common_args(p, A);
+    A.shot_begin = 0;
+    A.shot_count = 1;
+    A.mode = MODE_PRESET;
+    A.preset = d_ps;
+    A.preset_amps = d_amp;
+    A.preset_records = d_rec;
+    A.preset_ev = d_ev;
+    A.forced_records = s->forced_record ? d_forced : nullptr;
+    HIP_TRY(launch_with_args(p, L, p->dp_dev, A, st));
+    // the kernel writes the final amplitudes back to d_amp
+    HIP_TRY(hipMemcpyAsync(&ps, d_ps, sizeof(ps), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(recs.data(), d_rec, nrec, hipMemcpyDeviceToHost, st));
+    if (s->exp_vals && p->num_ev) HIP_TRY(hipMemcpyAsync(s->exp_vals, d_ev, p->num_ev * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->v, d_amp, namp * 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    s->gamma_re = ps.gamma[0];
+    s->gamma_im = ps.gamma[1];
+    for (int i = 0; i < 4; ++i) s->rng[i] = ps.rng[i];
+    s->active_k = ps.active_k;
+    s->discarded = ps.discarded;
+    s->next_noise_idx = ps.next_noise_idx;
+    s->forced_reachable = ps.forced_reachable;
+    s->forced_log_probability = ps.forced_log_probability;
+    s->dust_clamps = ps.dust_clamps;
+    for (uint32_t w = 0; w < p->words; ++w) {
+        s->px[w] = ps.px[w];
+        s->pz[w] = ps.pz[w];
+    }
+    if (s->meas_record) std::memcpy(s->meas_record, recs.data(), p->total_slots);
+    if (s->det_record) std::memcpy(s->det_record, recs.data() + p->total_slots, p->num_det);
+    if (s->obs_record) std::memcpy(s->obs_record, recs.data() + p->total_slots + p->num_det, p->num_obs);
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_sample(clifft_svm_program* const* programs, const uint64_t* shots, const uint64_t* seeds,
+                            uint32_t num_programs, uint32_t flags, const clifft_svm_records* outs, void* stream) {
+    if (!programs || !shots || !seeds || !outs) return fail(CLIFFT_E_INVALID, "null argument");
+    for (uint32_t i = 0; i < num_programs; ++i) {
+        int rc = clifft_svm_sample(programs[i], shots[i], seeds[i], flags, &outs[i], stream);
+        if (rc) return rc;
+    }
+    return CLIFFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,590 @@
+// svm_interp.h -- the per-shot bytecode interpreter run by one team.
+//
+// Every handler restates one reference opcode handler (citation on each,
+// paths relative to the reference repository).  Frame, gamma, rank and PRNG
+// updates are wavefront-uniform and done redundantly by every lane; the
+// amplitude work is delegated to the sweeps in svm_sweeps.h.
+#pragma once
+
+#include "svm_sweeps.h"
+
+namespace csvm {
+
+// Pauli frame of NW words held in scalar registers (used with NW = 1).
+template <int NW>
+struct FrameReg {
+    uint64_t x[NW];
+    uint64_t z[NW];
+    CSVM_INLINE void init(void*) {
+#pragma unroll
+        for (int i = 0; i < NW; ++i) x[i] = z[i] = 0;
+    }
+    CSVM_INLINE static uint64_t sel(const uint64_t (&m)[NW], uint32_t w) {
+        if (NW == 1) return m[0];
+        return w == 0 ? m[0] : m[NW - 1];
+    }
+    CSVM_INLINE static void xorw(uint64_t (&m)[NW], uint32_t w, uint64_t v) {
+        if (NW == 1 || w == 0)
+            m[0] ^= v;
+        else
+            m[NW - 1] ^= v;
+    }
+    CSVM_INLINE uint32_t gx(uint32_t q) const { return (uint32_t)(sel(x, q >> 6) >> (q & 63)) & 1u; }
+    CSVM_INLINE uint32_t gz(uint32_t q) const { return (uint32_t)(sel(z, q >> 6) >> (q & 63)) & 1u; }
+    CSVM_INLINE void fx(uint32_t q, uint32_t b) { xorw(x, q >> 6, (uint64_t)(b & 1u) << (q & 63)); }
+    CSVM_INLINE void fz(uint32_t q, uint32_t b) { xorw(z, q >> 6, (uint64_t)(b & 1u) << (q & 63)); }
+    CSVM_INLINE uint64_t wx(uint32_t w) const { return sel(x, w); }
+    CSVM_INLINE uint64_t wz(uint32_t w) const { return sel(z, w); }
+    CSVM_INLINE void xor_words(uint32_t w, uint64_t ex, uint64_t ez) {
+        xorw(x, w, ex);
+        xorw(z, w, ez);
+    }
+    CSVM_INLINE void set_words(uint32_t w, uint64_t vx, uint64_t vz) {
+        xorw(x, w, sel(x, w) ^ vx);
+        xorw(z, w, sel(z, w) ^ vz);
+    }
+};
+
+// Pauli frame of up to 16 words in this wavefront's LDS block (wider
+// programs).  Lane 0 writes, every lane reads after a wavefront fence.
+struct FrameLds {
+    __attribute__((address_space(3))) uint64_t* m;  // x[16] then z[16]
+    CSVM_INLINE static void wfence() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    CSVM_INLINE void init(void* base) {
+        m = (__attribute__((address_space(3))) uint64_t*)(base);
+        if ((threadIdx.x & 63u) == 0)
+            for (int i = 0; i < 32; ++i) m[i] = 0;
+        wfence();
+    }
+    CSVM_INLINE uint64_t wx(uint32_t w) const { return uni64(m[w]); }
+    CSVM_INLINE uint64_t wz(uint32_t w) const { return uni64(m[16 + w]); }
+    CSVM_INLINE uint32_t gx(uint32_t q) const { return (uint32_t)(wx(q >> 6) >> (q & 63)) & 1u; }
+    CSVM_INLINE uint32_t gz(uint32_t q) const { return (uint32_t)(wz(q >> 6) >> (q & 63)) & 1u; }
+    CSVM_INLINE void xor_words(uint32_t w, uint64_t ex, uint64_t ez) {
+        if ((threadIdx.x & 63u) == 0) {
+            m[w] ^= ex;
+            m[16 + w] ^= ez;
+        }
+        wfence();
+    }
+    CSVM_INLINE void fx(uint32_t q, uint32_t b) { xor_words(q >> 6, (uint64_t)(b & 1u) << (q & 63), 0); }
+    CSVM_INLINE void fz(uint32_t q, uint32_t b) { xor_words(q >> 6, 0, (uint64_t)(b & 1u) << (q & 63)); }
+    CSVM_INLINE void set_words(uint32_t w, uint64_t vx, uint64_t vz) {
+        if ((threadIdx.x & 63u) == 0) {
+            m[w] = vx;
+            m[16 + w] = vz;
+        }
+        wfence();
+    }
+};
+
+template <class F, bool BT, class AmpT>
+struct Interp {
+    TeamGeom g;
+    AmpT* amp;
+    lds_f64* red;
+    __attribute__((address_space(3))) uint8_t* meas;
+    __attribute__((address_space(3))) uint8_t* det;
+    __attribute__((address_space(3))) uint8_t* obs;
+    F f;
+    double g_re, g_im;
+    uint32_t k;
+    Rng rng;
+    uint32_t next_noise;
+    uint32_t discarded;
+    uint32_t reachable;
+    double logp;
+    uint32_t dust;
+    cptr<uint8_t> forced;
+
+    // ----------------------------------------------------------------- gamma
+    CSVM_INLINE void mul_phase(double re, double im) {
+        double a = g_re, b = g_im;
+        g_re = a * re - b * im;
+        g_im = a * im + b * re;
+    }
+    CSVM_INLINE void negate_gamma() {
+        g_re = -g_re;
+        g_im = -g_im;
+    }
+    // svm.h:177-186
+    CSVM_INLINE void scale_magnitude(double s) {
+        g_re *= s;
+        g_im *= s;
+        double gm = hypot(g_re, g_im);
+        if (gm > 1e100 || (gm < 1e-100 && gm > 0.0)) {
+            sweep_scale<BT>(amp, k, gm, g);
+            g_re /= gm;
+            g_im /= gm;
+        }
+    }
+    // svm_internal.h:65-79
+    CSVM_INLINE uint32_t sample_branch(double p0, double p1, double total) {
+        double eps = kDustEpsilon * total;
+        if (p1 <= eps) {
+            if (p1 > 0.0) dust++;
+            return 0;
+        }
+        if (p0 <= eps) {
+            if (p0 > 0.0) dust++;
+            return 1;
+        }
+        double r = rng_double(rng);
+        return (r * total < p0) ? 0u : 1u;
+    }
+    // svm_forced_kernels.cc:43-61
+    CSVM_INLINE bool forced_increment(double p0, double p1, double total, uint32_t b, double& inc) {
+        double eps = kDustEpsilon * total;
+        inc = 0.0;
+        if (p1 <= eps) {
+            if (p1 > 0.0) dust++;
+            return b == 0;
+        }
+        if (p0 <= eps) {
+            if (p0 > 0.0) dust++;
+            return b == 1;
+        }
+        inc = log((b ? p1 : p0) / total);
+        return true;
+    }
+
+    // ------------------------------------------------------------ records
+    CSVM_INLINE void rec_write(__attribute__((address_space(3))) uint8_t* r, uint32_t idx, uint32_t val) {
+        if ((threadIdx.x & 63u) == 0) r[idx] = (uint8_t)val;
+        FrameLds::wfence();
+    }
+    CSVM_INLINE uint32_t rec_read(const __attribute__((address_space(3))) uint8_t* r, uint32_t idx) {
+        return uni(r[idx]);
+    }
+
+    // ------------------------------------------------------------ frame ops
+    // svm_kernels.inl:67-116, 749-755
+    CSVM_INLINE void frame_cnot(uint32_t c, uint32_t t) {
+        uint32_t pxc = f.gx(c), pzt = f.gz(t);
+        f.fx(t, pxc);
+        f.fz(c, pzt);
+    }
+    CSVM_INLINE void frame_cz(uint32_t c, uint32_t t) {
+        uint32_t pxc = f.gx(c), pxt = f.gx(t);
+        if (pxc & pxt) negate_gamma();
+        f.fz(t, pxc);
+        f.fz(c, pxt);
+    }
+    CSVM_INLINE void frame_h(uint32_t v) {
+        uint32_t px = f.gx(v), pz = f.gz(v);
+        if (px & pz) negate_gamma();
+        if (px != pz) {
+            f.fx(v, 1);
+            f.fz(v, 1);
+        }
+    }
+    CSVM_INLINE void frame_s(uint32_t v, double sgn) {
+        uint32_t px = f.gx(v);
+        if (px) mul_phase(0.0, sgn);
+        f.fz(v, px);
+    }
+    CSVM_INLINE void frame_swap(uint32_t a, uint32_t b) {
+        uint32_t dx = f.gx(a) ^ f.gx(b), dz = f.gz(a) ^ f.gz(b);
+        f.fx(a, dx);
+        f.fx(b, dx);
+        f.fz(a, dz);
+        f.fz(b, dz);
+    }
+    CSVM_INLINE void set_x(uint32_t q, uint32_t b) { f.fx(q, f.gx(q) ^ b); }
+    CSVM_INLINE void set_z(uint32_t q, uint32_t b) { f.fz(q, f.gz(q) ^ b); }
+
+    // :1812-1866
+    CSVM_INLINE void apply_pauli(cptr<uint64_t> ex, cptr<uint64_t> ez, uint32_t words, uint32_t sign) {
+        uint32_t parity = 0;
+        for (uint32_t w = 0; w < words; ++w) parity += __popcll(f.wx(w) & ez[w]);
+        if (parity & 1u) negate_gamma();
+        for (uint32_t w = 0; w < words; ++w) f.xor_words(w, ex[w], ez[w]);
+        if (sign) negate_gamma();
+    }
+    // svm.h:239-248
+    CSVM_INLINE void draw_next_noise(cptr<DevProgram> P) {
+        uint32_t n = P->num_sites;
+        if (n == 0 || next_noise >= n) {
+            next_noise = kNoNoise;
+            return;
+        }
+        cptr<double> hz = cp(P->hazards);
+        double cur = next_noise == 0 ? 0.0 : hz[next_noise - 1];
+        double gap = -log(1.0 - rng_double(rng));
+        double target = cur + gap;
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (hz[mid] <= target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        next_noise = lo;
+    }
+    // :1886-1920
+    CSVM_INLINE void noise(cptr<DevProgram> P, uint32_t site) {
+        if (site != next_noise) return;
+        cptr<uint32_t> off = cp(P->site_chan_off);
+        cptr<double> prob = cp(P->chan_prob);
+        uint32_t c0 = off[site], c1 = off[site + 1];
+        double psum = 0.0;
+        for (uint32_t c = c0; c < c1; ++c) psum += prob[c];
+        double r = rng_double(rng) * psum;
+        double cum = 0.0;
+        for (uint32_t c = c0; c < c1; ++c) {
+            cum += prob[c];
+            if (r < cum) {
+                uint32_t W = P->words;
+                apply_pauli(cp(P->chan_x) + (size_t)c * W, cp(P->chan_z) + (size_t)c * W, W, 0);
+                break;
+            }
+        }
+        next_noise++;
+        draw_next_noise(P);
+    }
+    CSVM_INLINE uint32_t list_parity(cptr<uint32_t> off, cptr<uint32_t> tgt, uint32_t l) {
+        uint32_t par = 0;
+        uint32_t j1 = off[l + 1];
+        for (uint32_t j = off[l]; j < j1; ++j) par ^= meas[tgt[j]];
+        return uni(par);
+    }
+
+    // ------------------------------------------------------------ measurement
+    // Shared tail of the active measurements: outcome choice (sampled or
+    // forced), then the caller's compaction, rank decrement, renormalisation
+    // and frame reset.  Returns false if a forced outcome is unreachable.
+    CSVM_INLINE bool choose(double p0, double p1, double total, uint32_t frame_bit, uint32_t sign, uint32_t idx,
+                            bool is_forced, uint32_t& b, uint32_t& m_abs, uint32_t& rec) {
+        if (is_forced) {
+            rec = uni(forced[idx]);
+            m_abs = rec ^ sign;
+            b = m_abs ^ frame_bit;
+            double inc;
+            if (!forced_increment(p0, p1, total, b, inc)) {
+                reachable = 0;
+                return false;
+            }
+            logp += inc;
+        } else {
+            b = sample_branch(p0, p1, total);
+            m_abs = b ^ frame_bit;
+            rec = m_abs ^ sign;
+        }
+        return true;
+    }
+    // :1573-1627; svm_forced_kernels.cc:118-171
+    CSVM_INLINE bool meas_diagonal(uint32_t v, uint32_t idx, uint32_t sign, bool is_forced) {
+        uint32_t px = f.gx(v), pz = f.gz(v);
+        double2 pr = sweep_probs<BT>(amp, k, PROB_DIAG, 0u, g, red);
+        double total = pr.x + pr.y;
+        uint32_t b, m_abs, rec;
+        if (!choose(pr.x, pr.y, total, px, sign, idx, is_forced, b, m_abs, rec)) return false;
+        if (b == 1 && pz) negate_gamma();
+        if (b == 1) sweep_take_upper<BT>(amp, k, g);
+        k--;
+        double pb = b ? pr.y : pr.x;
+        if (is_forced || pb > 0.0) scale_magnitude(sqrt(total / pb));
+        set_x(v, m_abs);
+        set_z(v, 0);
+        rec_write(meas, idx, rec);
+        return true;
+    }
+    // :1634-1708; svm_forced_kernels.cc:173-229
+    CSVM_INLINE bool meas_interfere(uint32_t v, uint32_t idx, uint32_t sign, bool is_forced) {
+        uint32_t px = f.gx(v), pz = f.gz(v);
+        double2 pr = sweep_probs<BT>(amp, k, PROB_INTERFERE, 0u, g, red);
+        double total = pr.x + pr.y;
+        uint32_t bx, m_abs, rec;
+        if (!choose(pr.x, pr.y, total, pz, sign, idx, is_forced, bx, m_abs, rec)) return false;
+        sweep_fold<BT>(amp, k, bx ? -1.0 : 1.0, g);
+        if (px && m_abs) negate_gamma();
+        k--;
+        double pb = bx ? pr.y : pr.x;
+        if (is_forced || pb > 0.0) scale_magnitude(sqrt(total / pb));
+        set_x(v, m_abs);
+        set_z(v, 0);
+        rec_write(meas, idx, rec);
+        return true;
+    }
+    // :1728-1804; svm_forced_kernels.cc:231-304
+    CSVM_INLINE bool swap_meas_interfere(uint32_t fa, uint32_t t, uint32_t idx, uint32_t sign, bool is_forced) {
+        if (fa == t) return meas_interfere(t, idx, sign, is_forced);
+        frame_swap(fa, t);
+        uint32_t px = f.gx(t), pz = f.gz(t);
+        double2 pr = sweep_probs<BT>(amp, k, PROB_SWAP_INTERFERE, fa, g, red);
+        double total = pr.x + pr.y;
+        uint32_t bx, m_abs, rec;
+        if (!choose(pr.x, pr.y, total, pz, sign, idx, is_forced, bx, m_abs, rec)) return false;
+        sweep_swap_fold<BT>(amp, k, fa, bx ? -1.0 : 1.0, g);
+        k--;
+        double pb = bx ? pr.y : pr.x;
+        if (is_forced || pb > 0.0) scale_magnitude(sqrt(total / pb));
+        if (px && m_abs) negate_gamma();
+        set_x(t, m_abs);
+        set_z(t, 0);
+        rec_write(meas, idx, rec);
+        return true;
+    }
+
+    // :2031-2173
+    CSVM_INLINE double exp_val(cptr<DevProgram> P, uint32_t ci) {
+        uint32_t W = P->words;
+        cptr<uint64_t> mx = cp(P->ev_x) + (size_t)ci * W;
+        cptr<uint64_t> mz = cp(P->ev_z) + (size_t)ci * W;
+        uint32_t fsign = cp(P->ev_sign)[ci] ? 1u : 0u;
+        uint32_t parity = 0;
+        for (uint32_t w = 0; w < W; ++w) parity += __popcll(mx[w] & f.wz(w)) + __popcll(mz[w] & f.wx(w));
+        if (parity & 1u) fsign ^= 1u;
+        uint32_t full = k / 64, rem = k % 64;
+        bool dormant_x = false;
+        if (rem > 0 && full < W && (mx[full] & ~((1ULL << rem) - 1ULL))) dormant_x = true;
+        for (uint32_t w = (rem == 0 ? full : full + 1); w < W; ++w)
+            if (mx[w]) dormant_x = true;
+        if (dormant_x) return 0.0;
+        uint32_t xa = 0, za = 0;
+        if (k > 0) {
+            uint64_t am = (1ULL << k) - 1ULL;
+            xa = (uint32_t)(mx[0] & am);
+            za = (uint32_t)(mz[0] & am);
+        }
+        uint32_t yc = __popc(xa & za) & 3u;
+        double ph_r = (yc == 0) ? 1.0 : (yc == 2 ? -1.0 : 0.0);
+        double ph_i = (yc == 1) ? 1.0 : (yc == 3 ? -1.0 : 0.0);
+        if (fsign) {
+            ph_r = -ph_r;
+            ph_i = -ph_i;
+        }
+        double3 sums = sweep_exp_val<BT>(amp, k, xa, za, g, red);
+        double num_re = sums.x * ph_r - sums.y * ph_i;
+        double res = sums.z > 0.0 ? num_re / sums.z : 0.0;
+        if (res > 1.0) res = 1.0;
+        if (res < -1.0) res = -1.0;
+        return res;
+    }
+
+    // ------------------------------------------------------------- dispatch
+    // svm_kernels.inl:2183-2646.  Returns when the bytecode ends, a
+    // POSTSELECT discards the shot, or a forced outcome is unreachable.
+    CSVM_INLINE void run(cptr<DevProgram> Pg, double* ev_out) {
+        cptr<DevProgram> P0 = opaque(Pg);
+        cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(P0->bc));
+        const uint32_t n_instr = P0->n_instr;
+        for (uint32_t pc = 0; pc < n_instr; ++pc) {
+            const uint64_t w0 = code[4 * pc], w1 = code[4 * pc + 1];
+            const uint32_t op = (uint32_t)(w0 & 0xFF);
+            const uint32_t flags = (uint32_t)((w0 >> 16) & 0xFF);
+            const uint32_t a1 = (uint32_t)((w0 >> 32) & 0xFFFF);
+            const uint32_t a2 = (uint32_t)(w0 >> 48);
+            const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
+            const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
+            switch (op) {
+                case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
+                case CLIFFT_OP_FRAME_CZ: frame_cz(a1, a2); break;
+                case CLIFFT_OP_FRAME_H: frame_h(a1); break;
+                case CLIFFT_OP_FRAME_S: frame_s(a1, 1.0); break;
+                case CLIFFT_OP_FRAME_S_DAG: frame_s(a1, -1.0); break;
+                case CLIFFT_OP_FRAME_SWAP: frame_swap(a1, a2); break;
+                case CLIFFT_OP_ARRAY_CNOT:  // :124-252
+                    sweep_two_axis<BT>(amp, k, a1, a2, PERM_CNOT, g);
+                    frame_cnot(a1, a2);
+                    break;
+                case CLIFFT_OP_ARRAY_CZ:  // :255-365
+                    sweep_two_axis<BT>(amp, k, a1, a2, PERM_CZ, g);
+                    frame_cz(a1, a2);
+                    break;
+                case CLIFFT_OP_ARRAY_SWAP:  // :369-458
+                    sweep_two_axis<BT>(amp, k, a1, a2, PERM_SWAP, g);
+                    frame_swap(a1, a2);
+                    break;
+                case CLIFFT_OP_ARRAY_MULTI_CNOT: {  // :464-550
+                    uint64_t m = w1 & ((1ULL << k) - 1ULL);
+                    sweep_multi<BT>(amp, k, a1, (uint32_t)m, 0u, g);
+                    while (m) {
+                        uint32_t c = (uint32_t)__builtin_ctzll(m);
+                        m &= m - 1;
+                        frame_cnot(c, a1);
+                    }
+                    break;
+                }
+                case CLIFFT_OP_ARRAY_MULTI_CZ: {  // :559-635
+                    uint64_t m = w1 & ((1ULL << k) - 1ULL);
+                    sweep_multi<BT>(amp, k, a1, (uint32_t)m, 1u, g);
+                    while (m) {
+                        uint32_t t = (uint32_t)__builtin_ctzll(m);
+                        m &= m - 1;
+                        frame_cz(a1, t);
+                    }
+                    break;
+                }
+                case CLIFFT_OP_ARRAY_H:  // :639-744
+                    sweep_h<BT>(amp, k, a1, g);
+                    frame_h(a1);
+                    break;
+                case CLIFFT_OP_ARRAY_S:  // :854-858
+                    sweep_phase<BT>(amp, k, a1, 0.0, 1.0, g);
+                    frame_s(a1, 1.0);
+                    break;
+                case CLIFFT_OP_ARRAY_S_DAG:  // :862-866
+                    sweep_phase<BT>(amp, k, a1, 0.0, -1.0, g);
+                    frame_s(a1, -1.0);
+                    break;
+                case CLIFFT_OP_ARRAY_T:
+                case CLIFFT_OP_ARRAY_T_DAG:
+                case CLIFFT_OP_ARRAY_ROT: {  // :951-988, 1044-1061
+                    double zr = kInvSqrt2, zi = (op == CLIFFT_OP_ARRAY_T_DAG) ? -kInvSqrt2 : kInvSqrt2;
+                    if (op == CLIFFT_OP_ARRAY_ROT) {
+                        zr = __longlong_as_double((long long)w1);
+                        zi = __longlong_as_double((long long)code[4 * pc + 2]);
+                    }
+                    uint32_t px = f.gx(a1);
+                    if (a1 < k) sweep_phase<BT>(amp, k, a1, zr, px ? -zi : zi, g);
+                    if (px) mul_phase(zr, zi);
+                    break;
+                }
+                case CLIFFT_OP_ARRAY_U2: {  // :1091-1297
+                    cptr<DevProgram> P = opaque(Pg);
+                    cptr<clifft_u2_node> node = cp(P->u2) + i0;
+                    uint32_t s = (f.gz(a1) << 1) | f.gx(a1);
+                    mul_phase(node->gamma[s][0], node->gamma[s][1]);
+                    uint32_t out = node->out_states[s];
+                    set_x(a1, out & 1u);
+                    set_z(a1, (out >> 1) & 1u);
+                    if (a1 < k) sweep_u2<BT>(amp, k, a1, &node->matrices[s][0][0], g);
+                    break;
+                }
+                case CLIFFT_OP_ARRAY_U4: {  // :1310-1531
+                    cptr<DevProgram> P = opaque(Pg);
+                    uint32_t s = (f.gz(a2) << 3) | (f.gx(a2) << 2) | (f.gz(a1) << 1) | f.gx(a1);
+                    cptr<clifft_u4_entry> e = &cp(P->u4)[i0].entries[s];
+                    mul_phase(e->gamma[0], e->gamma[1]);
+                    uint32_t out = e->out_state;
+                    set_x(a1, out & 1u);
+                    set_z(a1, (out >> 1) & 1u);
+                    set_x(a2, (out >> 2) & 1u);
+                    set_z(a2, (out >> 3) & 1u);
+                    sweep_u4<BT>(amp, k, a1, a2, &e->matrix[0][0][0], g);
+                    break;
+                }
+                case CLIFFT_OP_EXPAND:
+                case CLIFFT_OP_EXPAND_T:
+                case CLIFFT_OP_EXPAND_T_DAG:
+                case CLIFFT_OP_EXPAND_ROT: {  // :876-945, 995-1034, 1065-1081
+                    double zr = kInvSqrt2, zi = (op == CLIFFT_OP_EXPAND_T_DAG) ? -kInvSqrt2 : kInvSqrt2;
+                    if (op == CLIFFT_OP_EXPAND_ROT) {
+                        zr = __longlong_as_double((long long)w1);
+                        zi = __longlong_as_double((long long)code[4 * pc + 2]);
+                    }
+                    const bool has_phase = op != CLIFFT_OP_EXPAND;
+                    uint32_t px = f.gx(a1);
+                    sweep_expand<BT>(amp, k, zr, px ? -zi : zi, has_phase ? 1u : 0u, g);
+                    k++;
+                    scale_magnitude(1.0 / sqrt(2.0));
+                    if (has_phase && px) mul_phase(zr, zi);
+                    break;
+                }
+                case CLIFFT_OP_MEAS_DORMANT_STATIC:  // :1540-1545, :2361-2369
+                    rec_write(meas, i0, (flags & CLIFFT_FLAG_IDENTITY) ? sign : (f.gx(a1) ^ sign));
+                    break;
+                case CLIFFT_OP_MEAS_DORMANT_RANDOM: {  // :1551-1566
+                    uint32_t m = rng_double(rng) < 0.5 ? 0u : 1u;
+                    if (f.gx(a1) && m) negate_gamma();
+                    set_x(a1, m);
+                    set_z(a1, 0);
+                    rec_write(meas, i0, m ^ sign);
+                    break;
+                }
+                case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL: meas_diagonal(a1, i0, sign, false); break;
+                case CLIFFT_OP_MEAS_ACTIVE_INTERFERE: meas_interfere(a1, i0, sign, false); break;
+                case CLIFFT_OP_SWAP_MEAS_INTERFERE: swap_meas_interfere(a1, a2, i0, sign, false); break;
+                case CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED: {  // svm_forced_kernels.cc:63-89
+                    uint32_t det_v = (flags & CLIFFT_FLAG_IDENTITY) ? sign : (f.gx(a1) ^ sign);
+                    if (uni(forced[i0]) != det_v) {
+                        reachable = 0;
+                        return;
+                    }
+                    rec_write(meas, i0, det_v);
+                    break;
+                }
+                case CLIFFT_OP_MEAS_DORMANT_RANDOM_FORCED: {  // svm_forced_kernels.cc:91-116
+                    uint32_t req = uni(forced[i0]);
+                    uint32_t m = req ^ sign;
+                    logp += log(0.5);
+                    if (f.gx(a1) && m) negate_gamma();
+                    set_x(a1, m);
+                    set_z(a1, 0);
+                    rec_write(meas, i0, req);
+                    break;
+                }
+                case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL_FORCED:
+                    if (!meas_diagonal(a1, i0, sign, true)) return;
+                    break;
+                case CLIFFT_OP_MEAS_ACTIVE_INTERFERE_FORCED:
+                    if (!meas_interfere(a1, i0, sign, true)) return;
+                    break;
+                case CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED:
+                    if (!swap_meas_interfere(a1, a2, i0, sign, true)) return;
+                    break;
+                case CLIFFT_OP_APPLY_PAULI:  // :1869-1880
+                    if (rec_read(meas, i1)) {
+                        cptr<DevProgram> P = opaque(Pg);
+                        uint32_t W = P->words;
+                        apply_pauli(cp(P->pauli_x) + (size_t)i0 * W, cp(P->pauli_z) + (size_t)i0 * W, W, cp(P->pauli_sign)[i0]);
+                    }
+                    break;
+                case CLIFFT_OP_NOISE:
+                    if (i0 == next_noise) noise(opaque(Pg), i0);
+                    break;
+                case CLIFFT_OP_NOISE_BLOCK: {  // :1926-1932
+                    uint32_t end = i0 + i1;
+                    if (next_noise >= i0 && next_noise < end) {
+                        cptr<DevProgram> P = opaque(Pg);
+                        while (next_noise >= i0 && next_noise < end) noise(P, next_noise);
+                    }
+                    break;
+                }
+                case CLIFFT_OP_READOUT_NOISE: {  // :1937-1959
+                    cptr<DevProgram> P = opaque(Pg);
+                    if (rng_double(rng) < cp(P->ro_prob)[i0]) {
+                        uint32_t mi = cp(P->ro_meas_idx)[i0];
+                        rec_write(meas, mi, rec_read(meas, mi) ^ 1u);
+                    }
+                    break;
+                }
+                case CLIFFT_OP_DETECTOR: {  // :1964-1979
+                    cptr<DevProgram> P = opaque(Pg);
+                    uint32_t par = ((flags & CLIFFT_FLAG_EXPECTED_ONE) ? 1u : 0u) ^ list_parity(cp(P->det_off), cp(P->det_tgt), i0);
+                    rec_write(det, i1, par);
+                    break;
+                }
+                case CLIFFT_OP_POSTSELECT: {  // :1984-2004
+                    cptr<DevProgram> P = opaque(Pg);
+                    uint32_t par = ((flags & CLIFFT_FLAG_EXPECTED_ONE) ? 1u : 0u) ^ list_parity(cp(P->det_off), cp(P->det_tgt), i0);
+                    rec_write(det, i1, 0);
+                    if (par) {
+                        discarded = 1;
+                        return;
+                    }
+                    break;
+                }
+                case CLIFFT_OP_OBSERVABLE: {  // :2007-2020
+                    cptr<DevProgram> P = opaque(Pg);
+                    rec_write(obs, i1, rec_read(obs, i1) ^ list_parity(cp(P->obs_off), cp(P->obs_tgt), i0));
+                    break;
+                }
+                case CLIFFT_OP_EXP_VAL: {  // :2031-2173
+                    double r = exp_val(opaque(Pg), i0);
+                    if (ev_out && g.lane == 0) ev_out[i1] = r;
+                    break;
+                }
+                default: break;
+            }
+        }
+    }
+};
+
+}  // namespace csvm

@@ -1,0 +1,247 @@
+// svm_kernels.hip -- team-interpreter kernels for gfx950 and their launchers.
+#include "svm_internal.h"
+#include "svm_interp.h"
+
+namespace csvm {
+
+// Frame kinds: one word in scalar registers (<= 64 qubits), or up to 16
+// words in LDS.
+enum : int { FRAME_W1 = 1, FRAME_LDS = 0 };
+
+template <int FK>
+struct FrameOf;
+template <>
+struct FrameOf<FRAME_W1> {
+    typedef FrameReg<1> type;
+};
+template <>
+struct FrameOf<FRAME_LDS> {
+    typedef FrameLds type;
+};
+
+// LDS carve-up of one workgroup:
+//   [teams_per_block x amp_elems x 16 B]  amplitudes (LDS tier only)
+//   [512 B]                               team reduction scratch (16 x 4 f64)
+//   [waves_per_block x rec_stride]        per-wavefront block: frame (LDS
+//                                         frame kind: 256 B) then records
+template <int FK, bool BT, bool GA>
+__device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag) {
+    typedef typename FrameOf<FK>::type F;
+    typedef typename std::conditional<GA, glb_d2, lds_d2>::type AmpT;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    cptr<ShotArgs> A = opaque(Ag);
+    const uint32_t waves_block = blockDim.x >> 6;
+    const uint32_t wave_in_block = uni(threadIdx.x >> 6);
+    const uint32_t wl = threadIdx.x & 63u;
+    Interp<F, BT, AmpT> S;
+    uint32_t teams_per_block, team_in_block;
+    if (BT) {
+        team_in_block = 0;
+        teams_per_block = 1;
+        S.g.lane = threadIdx.x;
+        S.g.size = blockDim.x;
+        S.g.wave = wave_in_block;
+        S.g.nwaves = waves_block;
+    } else {
+        team_in_block = wave_in_block;
+        teams_per_block = waves_block;
+        S.g.lane = wl;
+        S.g.size = 64;
+        S.g.wave = 0;
+        S.g.nwaves = 1;
+    }
+    const uint32_t amp_elems = A->amp_elems;
+    const uint32_t rec_stride = A->rec_stride;
+    unsigned char* p = smem;
+    unsigned char* lds_amp = p;
+    if (!GA) p += (size_t)teams_per_block * amp_elems * sizeof(dv2);
+    S.red = (lds_f64*)(p);
+    p += 16 * 4 * sizeof(double);
+    unsigned char* wblock = p + (size_t)wave_in_block * rec_stride;
+    unsigned char* rec = wblock + (FK == FRAME_LDS ? 256 : 0);
+    typedef __attribute__((address_space(3))) uint8_t lds_u8;
+    {
+        cptr<DevProgram> P = opaque(Pg);
+        S.meas = (lds_u8*)(rec);
+        S.det = S.meas + P->total_meas_slots;
+        S.obs = S.det + P->num_det;
+    }
+    const uint64_t team_global = (uint64_t)blockIdx.x * teams_per_block + team_in_block;
+    const uint64_t nteams = (uint64_t)gridDim.x * teams_per_block;
+    if (GA)
+        S.amp = (AmpT*)(A->gscratch + team_global * amp_elems);
+    else
+        S.amp = (AmpT*)((dv2*)lds_amp + (size_t)team_in_block * amp_elems);
+
+    const uint64_t shot_count = A->shot_count;
+    for (uint64_t s = team_global; s < shot_count; s += nteams) {
+        A = opaque(Ag);
+        cptr<DevProgram> P = opaque(Pg);
+        const uint64_t shot = A->shot_begin + s;
+        const uint32_t mode = A->mode;
+        // ---- per-shot reset (svm_state.cc:220-267) ----
+        for (uint32_t i = wl; i < rec_stride / 4; i += 64) reinterpret_cast<uint32_t*>(wblock)[i] = 0u;
+        FrameLds::wfence();
+        S.f.init(wblock);
+        S.dust = 0;
+        S.discarded = 0;
+        S.reachable = 1;
+        S.logp = 0.0;
+        S.forced = nullptr;
+        if (mode == MODE_PRESET) {
+            cptr<PresetState> ps = cp(A->preset);
+            S.g_re = ps->gamma[0];
+            S.g_im = ps->gamma[1];
+            S.rng.s0 = ps->rng[0];
+            S.rng.s1 = ps->rng[1];
+            S.rng.s2 = ps->rng[2];
+            S.rng.s3 = ps->rng[3];
+            S.k = ps->active_k;
+            S.next_noise = ps->next_noise_idx;
+            S.logp = ps->forced_log_probability;
+            S.reachable = ps->forced_reachable;
+            for (uint32_t w = 0; w < P->words; ++w) S.f.set_words(w, ps->px[w], ps->pz[w]);
+            uint32_t n = 1u << P->peak_rank;
+            for (uint32_t i = S.g.lane; i < n; i += S.g.size) S.amp[i] = cp(A->preset_amps)[i];
+            uint32_t nrec = P->total_meas_slots + P->num_det + P->num_obs;
+            for (uint32_t i = wl; i < nrec; i += 64) rec[i] = cp(A->preset_records)[i];
+            FrameLds::wfence();
+            S.forced = cp(A->forced_records);
+        } else {
+            S.g_re = 1.0;
+            S.g_im = 0.0;
+            S.k = 0;
+            S.rng = rng_for_shot(A->seed, shot);
+            if (S.g.lane == 0) S.amp[0] = mk(1.0, 0.0);
+            S.next_noise = 0;
+            S.draw_next_noise(P);
+            if (mode == MODE_FORCED) S.forced = cp(A->forced_records) + s * P->num_meas;
+        }
+        team_sync<BT>();
+
+        double* ev_row = nullptr;
+        if (mode == MODE_PRESET) {
+            ev_row = A->preset_ev;
+        } else if (A->out_ev) {
+            ev_row = A->out_ev + s * P->num_ev;
+            if (S.g.lane == 0)
+                for (uint32_t i = 0; i < P->num_ev; ++i) ev_row[i] = 0.0;
+        }
+
+        S.run(Pg, ev_row);
+
+        A = opaque(Ag);
+        P = opaque(Pg);
+        if (mode == MODE_PRESET) {
+            uint32_t n = 1u << P->peak_rank;
+            for (uint32_t i = S.g.lane; i < n; i += S.g.size) A->preset_amps[i] = S.amp[i];
+        }
+        // ---- outputs (wave 0 of the team) ----
+        if (S.g.wave == 0) {
+            if (mode == MODE_PRESET) {
+                PresetState* ps = A->preset;
+                uint32_t nrec = P->total_meas_slots + P->num_det + P->num_obs;
+                for (uint32_t i = wl; i < nrec; i += 64) A->preset_records[i] = rec[i];
+                uint32_t W = P->words;
+                uint64_t fx[16], fz[16];
+                for (uint32_t w = 0; w < 16; ++w) {
+                    fx[w] = w < W ? S.f.wx(w) : 0;
+                    fz[w] = w < W ? S.f.wz(w) : 0;
+                }
+                if (wl == 0) {
+                    ps->gamma[0] = S.g_re;
+                    ps->gamma[1] = S.g_im;
+                    ps->rng[0] = S.rng.s0;
+                    ps->rng[1] = S.rng.s1;
+                    ps->rng[2] = S.rng.s2;
+                    ps->rng[3] = S.rng.s3;
+                    ps->active_k = S.k;
+                    ps->discarded = S.discarded;
+                    ps->next_noise_idx = S.next_noise;
+                    ps->forced_reachable = S.reachable;
+                    ps->forced_log_probability = S.logp;
+                    ps->dust_clamps += S.dust;
+                    for (uint32_t w = 0; w < W; ++w) {
+                        ps->px[w] = fx[w];
+                        ps->pz[w] = fz[w];
+                    }
+                }
+            } else {
+                const bool packed = (A->flags & CLIFFT_REC_PACKED) != 0;
+                const uint32_t n_exp = P->n_expected_obs;
+                cptr<uint8_t> expo = cp(P->expected_obs);
+                auto emit = [&](uint8_t* out, const lds_u8* src, uint32_t n, bool xor_expected) {
+                    if (!out || n == 0) return;
+                    if (packed) {
+                        uint32_t nb = (n + 7) / 8;
+                        for (uint32_t j = wl; j < nb; j += 64) {
+                            uint32_t byte = 0;
+                            for (uint32_t b = 0; b < 8; ++b) {
+                                uint32_t i = 8 * j + b;
+                                if (i < n) {
+                                    uint32_t v = src[i];
+                                    if (xor_expected && i < n_exp) v ^= expo[i] ? 1u : 0u;
+                                    byte |= (v & 1u) << b;
+                                }
+                            }
+                            out[s * nb + j] = (uint8_t)byte;
+                        }
+                    } else {
+                        for (uint32_t i = wl; i < n; i += 64) {
+                            uint32_t v = src[i];
+                            if (xor_expected && i < n_exp) v ^= expo[i] ? 1u : 0u;
+                            out[s * n + i] = (uint8_t)v;
+                        }
+                    }
+                };
+                emit(A->out_meas, S.meas, P->num_meas, false);
+                emit(A->out_det, S.det, P->num_det, false);
+                emit(A->out_obs, S.obs, P->num_obs, true);
+                if (wl == 0) {
+                    if (A->out_disc) A->out_disc[s] = (uint8_t)S.discarded;
+                    if (mode == MODE_FORCED && A->out_logp) A->out_logp[s] = S.reachable ? S.logp : kUnreachable;
+                    if (S.dust && A->dust_counter) atomicAdd(A->dust_counter, (unsigned long long)S.dust);
+                }
+            }
+        }
+        team_sync<BT>();
+    }
+}
+
+template <int FK, bool BT, bool GA>
+__global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram* P, const ShotArgs* A) {
+    shot_body<FK, BT, GA>(cp(P), cp(A));
+}
+
+// ---------------------------------------------------------------------------
+// Launch dispatch over the template parameters.
+// ---------------------------------------------------------------------------
+template <int FK, bool BT, bool GA>
+static hipError_t launch_one(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
+    auto kern = shot_kernel<FK, BT, GA>;
+    static bool attr_done = false;
+    if (!attr_done) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_done = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
+    return hipGetLastError();
+}
+
+template <int FK>
+static hipError_t launch_fk(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
+    if (L.block_team)
+        return L.global_amps ? launch_one<FK, true, true>(L, P, A, st) : launch_one<FK, true, false>(L, P, A, st);
+    return L.global_amps ? launch_one<FK, false, true>(L, P, A, st) : launch_one<FK, false, false>(L, P, A, st);
+}
+
+hipError_t launch_shots(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
+    switch (L.nw) {
+        case 1: return launch_fk<FRAME_W1>(L, P, A, st);
+        default: return launch_fk<FRAME_LDS>(L, P, A, st);
+    }
+}
+
+}  // namespace csvm

@@ -1,0 +1,331 @@
+// svm_sweeps.h -- amplitude sweeps and reductions of the team interpreter.
+//
+// Each sweep is a non-inlined device function taking only what it touches
+// (amplitude pointer in its own address space, rank, axes, coefficients),
+// so the dispatch loop keeps a small live register set.  AmpPtr is either an
+// LDS pointer (address space 3: ds_read/ds_write) or a global pointer
+// (address space 1: global_load/global_store).
+#pragma once
+
+#include "svm_device.h"
+
+namespace csvm {
+
+typedef __attribute__((address_space(3))) dv2 lds_d2;
+typedef __attribute__((address_space(1))) dv2 glb_d2;
+typedef __attribute__((address_space(3))) double lds_f64;
+
+#define CSVM_SWEEP __device__ __forceinline__
+
+// Team synchronisation: wavefront-level (one-wave teams) or workgroup barrier.
+template <bool BT>
+CSVM_INLINE void team_sync() {
+    if (BT) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+struct TeamGeom {
+    uint32_t lane;    // lane within team
+    uint32_t size;    // threads in team
+    uint32_t wave;    // wavefront within team
+    uint32_t nwaves;  // wavefronts in team
+};
+
+// Team sum of N doubles, identical bits on every lane: lane butterfly, then
+// (workgroup teams) waves summed in index order through LDS scratch `red`
+// (16 x 4 doubles), fenced by a barrier on both sides.
+template <bool BT, int N>
+CSVM_INLINE void team_reduce(double (&v)[N], const TeamGeom& g, lds_f64* red) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += __shfl_xor(v[i], off, 64);
+    }
+    if (BT) {
+        if (g.nwaves > 1) {
+            __syncthreads();
+            if ((g.lane & 63u) == 0) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) red[g.wave * 4 + i] = v[i];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double acc = red[i];
+                for (uint32_t w = 1; w < g.nwaves; ++w) acc += red[w * 4 + i];
+                v[i] = acc;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = unid(v[i]);
+}
+
+// Arguments of non-inlined device functions arrive in vector registers;
+// re-assert their uniformity so loop control and coefficient loads stay
+// scalar.
+CSVM_INLINE TeamGeom uni_geom(TeamGeom g) {
+    TeamGeom r;
+    r.lane = g.lane;
+    r.size = uni(g.size);
+    r.wave = uni(g.wave);
+    r.nwaves = uni(g.nwaves);
+    return r;
+}
+
+// --------------------------------------------------------------------------
+// Permutation / sign sweeps (svm_kernels.inl:124-635)
+// --------------------------------------------------------------------------
+enum : uint32_t { PERM_CNOT = 0, PERM_CZ = 1, PERM_SWAP = 2 };
+
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_two_axis(AmpT* amp, uint32_t k, uint32_t a, uint32_t b, uint32_t kind, TeamGeom g) {
+    k = uni(k); a = uni(a); b = uni(b); kind = uni(kind); g = uni_geom(g);
+    uint32_t n = 1u << (k - 2), ab = 1u << a, bb = 1u << b;
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    if (kind == PERM_CNOT) {  // a = control, b = target
+        for (uint32_t i = g.lane; i < n; i += g.size) {
+            uint32_t b0 = insert_two_zero_bits(i, lo, hi) | ab, b1 = b0 | bb;
+            dv2 x = amp[b0], y = amp[b1];
+            amp[b0] = y;
+            amp[b1] = x;
+        }
+    } else if (kind == PERM_CZ) {
+        for (uint32_t i = g.lane; i < n; i += g.size) {
+            uint32_t idx = insert_two_zero_bits(i, lo, hi) | ab | bb;
+            amp[idx] = cneg(amp[idx]);
+        }
+    } else {
+        for (uint32_t i = g.lane; i < n; i += g.size) {
+            uint32_t base = insert_two_zero_bits(i, lo, hi);
+            dv2 x = amp[base | ab], y = amp[base | bb];
+            amp[base | ab] = y;
+            amp[base | bb] = x;
+        }
+    }
+    team_sync<BT>();
+}
+
+// MULTI_CNOT (is_cz = 0): swap the target pair where parity(idx & mask) is
+// odd.  MULTI_CZ (is_cz = 1): negate entries with the control set and odd
+// parity(idx & mask).  `mask` is already restricted to active axes.
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_multi(AmpT* amp, uint32_t k, uint32_t axis, uint32_t mask, uint32_t is_cz, TeamGeom g) {
+    k = uni(k); axis = uni(axis); mask = uni(mask); is_cz = uni(is_cz); g = uni_geom(g);
+    uint32_t half = 1u << (k - 1), ab = 1u << axis;
+    if (!is_cz) {
+        for (uint32_t i = g.lane; i < half; i += g.size) {
+            uint32_t a = insert_zero_bit(i, axis);
+            if (__popc(a & mask) & 1u) {
+                dv2 x = amp[a], y = amp[a | ab];
+                amp[a] = y;
+                amp[a | ab] = x;
+            }
+        }
+    } else {
+        for (uint32_t i = g.lane; i < half; i += g.size) {
+            uint32_t a = insert_zero_bit(i, axis) | ab;
+            if (__popc(a & mask) & 1u) amp[a] = cneg(amp[a]);
+        }
+    }
+    team_sync<BT>();
+}
+
+// --------------------------------------------------------------------------
+// Single-axis sweeps: H butterfly (:639-744), diag(1, ph) (:764-850),
+// dense 2x2 (:1091-1297)
+// --------------------------------------------------------------------------
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_h(AmpT* amp, uint32_t k, uint32_t v, TeamGeom g) {
+    k = uni(k); v = uni(v); g = uni_geom(g);
+    uint32_t half = 1u << (k - 1), vb = 1u << v;
+    for (uint32_t i = g.lane; i < half; i += g.size) {
+        uint32_t i0 = insert_zero_bit(i, v), i1 = i0 | vb;
+        dv2 a = amp[i0], b = amp[i1];
+        amp[i0] = cscale(cadd(a, b), kInvSqrt2);
+        amp[i1] = cscale(csub(a, b), kInvSqrt2);
+    }
+    team_sync<BT>();
+}
+
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_phase(AmpT* amp, uint32_t k, uint32_t v, double pr, double pi, TeamGeom g) {
+    k = uni(k); v = uni(v); pr = unid(pr); pi = unid(pi); g = uni_geom(g);
+    uint32_t half = 1u << (k - 1), vb = 1u << v;
+    dv2 ph = mk(pr, pi);
+    for (uint32_t i = g.lane; i < half; i += g.size) {
+        uint32_t idx = insert_zero_bit(i, v) | vb;
+        amp[idx] = cmul(amp[idx], ph);
+    }
+    team_sync<BT>();
+}
+
+// m: row-major {a, b, c, d} as (re, im) pairs
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_u2(AmpT* amp, uint32_t k, uint32_t v, cptr<double> m, TeamGeom g) {
+    k = uni(k); v = uni(v); g = uni_geom(g);
+    dv2 m00 = mk(m[0], m[1]), m01 = mk(m[2], m[3]);
+    dv2 m10 = mk(m[4], m[5]), m11 = mk(m[6], m[7]);
+    uint32_t half = 1u << (k - 1), vb = 1u << v;
+    for (uint32_t i = g.lane; i < half; i += g.size) {
+        uint32_t i0 = insert_zero_bit(i, v), i1 = i0 | vb;
+        dv2 a = amp[i0], b = amp[i1];
+        amp[i0] = cadd(cmul(m00, a), cmul(m01, b));
+        amp[i1] = cadd(cmul(m10, a), cmul(m11, b));
+    }
+    team_sync<BT>();
+}
+
+// Dense 4x4 on quads {base, |lo, |hi, |lo|hi} (:1310-1531).  The matrix is
+// re-read (scalar loads) row by row inside the loop rather than held in 64
+// registers.
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_u4(AmpT* amp, uint32_t k, uint32_t lo, uint32_t hi, cptr<double> m, TeamGeom g) {
+    k = uni(k); lo = uni(lo); hi = uni(hi); g = uni_geom(g);
+    uint32_t n = 1u << (k - 2), lb = 1u << lo, hb = 1u << hi;
+    for (uint32_t i = g.lane; i < n; i += g.size) {
+        uint32_t base = insert_two_zero_bits(i, lo, hi);
+        uint32_t ix0 = base, ix1 = base | lb, ix2 = base | hb, ix3 = base | lb | hb;
+        dv2 in0 = amp[ix0], in1 = amp[ix1], in2 = amp[ix2], in3 = amp[ix3];
+        dv2 o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cptr<double> row = opaque(m) + 8 * r;
+            dv2 acc = cmul(mk(row[0], row[1]), in0);
+            acc = cadd(acc, cmul(mk(row[2], row[3]), in1));
+            acc = cadd(acc, cmul(mk(row[4], row[5]), in2));
+            acc = cadd(acc, cmul(mk(row[6], row[7]), in3));
+            o[r] = acc;
+        }
+        amp[ix0] = o[0];
+        amp[ix1] = o[1];
+        amp[ix2] = o[2];
+        amp[ix3] = o[3];
+    }
+    team_sync<BT>();
+}
+
+// EXPAND: amp[i + 2^k] = amp[i] * ph for i < 2^k (:876-945)
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_expand(AmpT* amp, uint32_t k, double pr, double pi, uint32_t has_phase, TeamGeom g) {
+    k = uni(k); pr = unid(pr); pi = unid(pi); has_phase = uni(has_phase); g = uni_geom(g);
+    uint32_t half = 1u << k;
+    dv2 ph = mk(pr, pi);
+    if (has_phase) {
+        for (uint32_t i = g.lane; i < half; i += g.size) amp[i + half] = cmul(amp[i], ph);
+    } else {
+        for (uint32_t i = g.lane; i < half; i += g.size) amp[i + half] = amp[i];
+    }
+    team_sync<BT>();
+}
+
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_scale(AmpT* amp, uint32_t k, double s, TeamGeom g) {
+    k = uni(k); s = unid(s); g = uni_geom(g);
+    uint32_t n = 1u << k;
+    for (uint32_t i = g.lane; i < n; i += g.size) amp[i] = cscale(amp[i], s);
+    team_sync<BT>();
+}
+
+// --------------------------------------------------------------------------
+// Measurement reductions and compaction (:1573-1804)
+// --------------------------------------------------------------------------
+enum : uint32_t { PROB_DIAG = 0, PROB_INTERFERE = 1, PROB_SWAP_INTERFERE = 2 };
+
+// Returns (p0, p1) of the branch pair on axis k-1 (or the swapped mapping
+// for SWAP_MEAS_INTERFERE with from-axis fa).
+template <bool BT, class AmpT>
+CSVM_SWEEP double2 sweep_probs(AmpT* amp, uint32_t k, uint32_t kind, uint32_t fa, TeamGeom g, lds_f64* red) {
+    k = uni(k); kind = uni(kind); fa = uni(fa); g = uni_geom(g);
+    uint32_t half = 1u << (k - 1);
+    double v[2] = {0.0, 0.0};
+    if (kind == PROB_DIAG) {
+        for (uint32_t i = g.lane; i < half; i += g.size) {
+            dv2 a = amp[i], b = amp[i + half];
+            v[0] += a.x * a.x + a.y * a.y;
+            v[1] += b.x * b.x + b.y * b.y;
+        }
+    } else if (kind == PROB_INTERFERE) {
+        for (uint32_t i = g.lane; i < half; i += g.size) {
+            dv2 a = amp[i], b = amp[i + half];
+            dv2 s = cadd(a, b), d = csub(a, b);
+            v[0] += s.x * s.x + s.y * s.y;
+            v[1] += d.x * d.x + d.y * d.y;
+        }
+    } else {
+        uint32_t t = k - 1, fb = 1u << fa;
+        for (uint32_t idx = g.lane; idx < half; idx += g.size) {
+            uint32_t bf = (idx >> fa) & 1u, base = (idx & ~fb) | (bf << t);
+            dv2 a = amp[base], b = amp[base | fb];
+            dv2 s = cadd(a, b), d = csub(a, b);
+            v[0] += s.x * s.x + s.y * s.y;
+            v[1] += d.x * d.x + d.y * d.y;
+        }
+    }
+    team_reduce<BT, 2>(v, g, red);
+    return make_double2(v[0], v[1]);
+}
+
+// Diagonal outcome b=1: keep the upper half (copy into the lower half).
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_take_upper(AmpT* amp, uint32_t k, TeamGeom g) {
+    k = uni(k); g = uni_geom(g);
+    uint32_t half = 1u << (k - 1);
+    for (uint32_t i = g.lane; i < half; i += g.size) amp[i] = amp[i + half];
+    team_sync<BT>();
+}
+
+// Interfere fold v'[i] = (v[i] +/- v[i+half]) / sqrt(2) (:1672-1687)
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_fold(AmpT* amp, uint32_t k, double s, TeamGeom g) {
+    k = uni(k); s = unid(s); g = uni_geom(g);
+    uint32_t half = 1u << (k - 1);
+    for (uint32_t i = g.lane; i < half; i += g.size) {
+        dv2 a = amp[i], b = amp[i + half];
+        amp[i] = mk((a.x + s * b.x) * kInvSqrt2, (a.y + s * b.y) * kInvSqrt2);
+    }
+    team_sync<BT>();
+}
+
+// SWAP_MEAS_INTERFERE fold with the swapped index mapping (:1768-1780).
+// Output idx < 2^t reads (base, base|f), base = (idx & ~f) | (bit_f(idx) << t).
+// Each lane owns the pair of outputs {i0, i0|f} (bit f of i0 clear); their
+// sources {i0, i0|f, i0|2^t, i0|2^t|f} belong to no other lane, so reading
+// all four before writing reproduces the serial in-place result.
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_swap_fold(AmpT* amp, uint32_t k, uint32_t fa, double s, TeamGeom g) {
+    k = uni(k); fa = uni(fa); s = unid(s); g = uni_geom(g);
+    uint32_t t = k - 1, quarter = 1u << (t - 1), fb = 1u << fa, tb = 1u << t;
+    for (uint32_t j = g.lane; j < quarter; j += g.size) {
+        uint32_t i0 = insert_zero_bit(j, fa), i1 = i0 | fb;
+        dv2 a0 = amp[i0], a1 = amp[i1], b0 = amp[i0 | tb], b1 = amp[i1 | tb];
+        amp[i0] = mk((a0.x + s * a1.x) * kInvSqrt2, (a0.y + s * a1.y) * kInvSqrt2);
+        amp[i1] = mk((b0.x + s * b1.x) * kInvSqrt2, (b0.y + s * b1.y) * kInvSqrt2);
+    }
+    team_sync<BT>();
+}
+
+// EXP_VAL sums (:2139-2152): returns (Re num, Im num, denominator).
+template <bool BT, class AmpT>
+CSVM_SWEEP double3 sweep_exp_val(AmpT* amp, uint32_t k, uint32_t xa, uint32_t za, TeamGeom g, lds_f64* red) {
+    k = uni(k); xa = uni(xa); za = uni(za); g = uni_geom(g);
+    uint32_t dim = 1u << k;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (uint32_t j = g.lane; j < dim; j += g.size) {
+        dv2 a = amp[j], b = amp[j ^ xa];
+        v[2] += a.x * a.x + a.y * a.y;
+        double zs = (__popc(j & za) & 1u) ? -1.0 : 1.0;
+        double cr = zs * a.x, ci = zs * a.y;
+        v[0] += b.x * cr + b.y * ci;
+        v[1] += b.x * ci - b.y * cr;
+    }
+    team_reduce<BT, 3>(v, g, red);
+    return make_double3(v[0], v[1], v[2]);
+}
+
+}  // namespace csvm

@@ -1,0 +1,131 @@
+"""Device executor vs CPU oracle on the same programs and seeds.
+
+Records (measurements, detectors, observables) must be bit-exact: under
+forced outcomes exactly, and in sampled mode because both sides draw from
+the same per-shot xoshiro256++ streams (a knife-edge comparison flip from a
+different reduction order has probability ~1e-16 per draw).  Amplitudes,
+gamma, expectation values and log-probabilities agree to 1e-12 absolute.
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from randprog import random_measure_only_program, random_program
+
+pytestmark = pytest.mark.gpu
+
+ATOL = 1e-12
+
+
+def announce(*a):
+    print("CASE", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def svm():
+    import clifft_amd.svm as svm
+    from clifft_amd import _lib
+    assert _lib.backend() == "hip:gfx950", _lib.backend()
+    return svm
+
+
+# (num_qubits, peak): wave-team LDS, workgroup-team LDS, global-memory tier,
+# and a >64-qubit frame held in LDS
+SHAPES = [(8, 3), (12, 6), (16, 8), (20, 10), (24, 12), (24, 13), (24, 14), (30, 15), (100, 6), (130, 9)]
+
+
+@pytest.mark.parametrize("nq,peak", SHAPES)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_sample_bit_exact(svm, orc, nq, peak, seed):
+    shots = 64 if peak >= 12 else 300
+    announce("sample", nq, peak, seed, shots)
+    p = random_program(1000 * seed + peak, num_qubits=nq, peak=peak, length=160 + 10 * peak)
+    res = svm.sample(p, shots, seed=seed)
+    m, d, o, e, _ = orc.sample(p, shots, seed=seed, per_shot=True)
+    assert np.array_equal(res.measurements, m)
+    assert np.array_equal(res.detectors, d)
+    assert np.array_equal(res.observables, o)
+    np.testing.assert_allclose(res.exp_vals, e, atol=ATOL, rtol=0)
+
+
+@pytest.mark.parametrize("peak", [4, 9, 14])
+def test_packed_records(svm, peak):
+    announce("packed", peak)
+    p = random_program(77 + peak, num_qubits=20, peak=peak, length=200)
+    a = svm.sample(p, 100, seed=5)
+    b = svm.sample(p, 100, seed=5, packed=True)
+    for x, y in ((a.measurements, b.measurements), (a.detectors, b.detectors), (a.observables, b.observables)):
+        assert np.array_equal(np.packbits(x, axis=1, bitorder="little"), y)
+
+
+@pytest.mark.parametrize("nq,peak", [(8, 4), (12, 7), (16, 10), (20, 14)])
+def test_record_log_probs(svm, orc, nq, peak):
+    announce("forced", nq, peak)
+    p = random_measure_only_program(peak, num_qubits=nq, peak=peak, length=120)
+    if p.num_measurements == 0:
+        pytest.skip("no measurements drawn")
+    recs, *_ = orc.sample(p, 40, seed=3)
+    rng = np.random.default_rng(peak)
+    flipped = recs.copy()
+    flipped[::3, rng.integers(0, p.num_measurements)] ^= 1  # some become unreachable
+    allrec = np.concatenate([recs, flipped])
+    gpu = svm.record_probabilities(p, allrec)
+    ref = orc.record_log_probs(p.forced(), allrec)
+    unreach_g, unreach_r = gpu < -1e300, ref < -1e300
+    assert np.array_equal(unreach_g, unreach_r)
+    np.testing.assert_allclose(gpu[~unreach_g], ref[~unreach_r], atol=1e-10, rtol=1e-12)
+
+
+@pytest.mark.parametrize("nq,peak", [(10, 5), (16, 9), (18, 14)])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_execute_preset_state(svm, orc, nq, peak, seed):
+    """Single-shot execute from a preset state: amplitudes, gamma, frame,
+    records and PRNG state after the shot (tests/test_svm.cc:28-67 style)."""
+    announce("execute", nq, peak, seed)
+    p = random_program(seed * 31 + peak, num_qubits=nq, peak=peak, length=150, noise=False)
+    rng = np.random.default_rng(seed)
+    o = orc.State.for_program(p, seed=seed)
+    g = svm.State.for_program(p, seed=seed)
+    o.next_noise_idx = 0xFFFFFFFF
+    g.next_noise_idx = 0xFFFFFFFF
+    svm.execute(p, g)
+    import ctypes
+    from oracle import oracle as orcmod
+    orcmod.execute(p, o)
+    assert g.active_k == o.active_k
+    n = 1 << g.active_k
+    np.testing.assert_allclose(g.v[:n], o.v[:n], atol=ATOL, rtol=0)
+    assert abs(g.gamma - o.gamma) < ATOL
+    assert int(g.p_x[0]) == int(o.p_x[0]) and int(g.p_z[0]) == int(o.p_z[0])
+    assert np.array_equal(g.meas_record, o.meas_record)
+    np.testing.assert_allclose(g.exp_vals, o.exp_vals, atol=ATOL, rtol=0)
+    assert list(map(int, g.rng)) == list(map(int, o.rng))
+
+
+def test_survivors_counts(svm, orc):
+    announce("survivors")
+    p = random_program(5, num_qubits=12, peak=5, length=250, postselect=True)
+    if not p.has_postselection:
+        pytest.skip("no postselect drawn")
+    g = svm.sample_survivors(p, 500, seed=11, keep_records=True)
+    r = orc.sample_survivors(p, 500, seed=11)
+    assert g.passed_shots == r["passed"]
+    assert g.logical_errors == r["logical_errors"]
+    assert np.array_equal(g.observable_ones, r["observable_ones"])
+    assert np.array_equal(g.measurements, r["measurements"])
+    assert np.array_equal(g.detectors, r["detectors"])
+
+
+def test_reference_stream_first_shot(svm, orc):
+    """Shot 0 of the device equals the first shot of the reference's single
+    stream (svm.cc:210-226)."""
+    announce("first-shot")
+    p = random_program(8, num_qubits=10, peak=5, length=200)
+    m, d, o, e, _ = orc.sample(p, 1, seed=1234, per_shot=False)
+    res = svm.sample(p, 1, seed=1234)
+    assert np.array_equal(res.measurements, m) and np.array_equal(res.detectors, d)
