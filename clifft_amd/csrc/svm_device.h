@@ -128,6 +128,17 @@ CSVM_INLINE cptr<T> cp(const T* p) {
 // Hide a table pointer's provenance from loop-invariant code motion so table
 // fields are loaded where they are used instead of pinning registers across
 // the interpreter loop.
+// Byte read from a constant-space table through the aligned dword that
+// contains it.  Scalar loads require a dword-aligned base address; forming
+// base + byte_index and reusing it as the base of a later scalar load would
+// fetch from a misaligned address, so byte reads never create such a base.
+// Tables are padded to at least 8 bytes, so the dword stays in bounds.
+CSVM_INLINE uint32_t load_u8(cptr<uint8_t> base, uint32_t idx) {
+    uint64_t a = (uint64_t)(base) + idx;
+    cptr<uint32_t> w = (cptr<uint32_t>)(a & ~3ull);
+    return (*w >> (8u * (uint32_t)(a & 3ull))) & 0xFFu;
+}
+
 CSVM_INLINE uint64_t uni64_(uint64_t x) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
     uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));

@@ -284,12 +284,11 @@ static constexpr uint32_t kArgSlots = 64;
 namespace {
 
 int frame_nw(uint32_t words) {
-    return words <= 1 ? 1 : 0;  // 0 = LDS-resident frame
+    return words <= 1 ? 1 : 0;  // 0 = frame spread over the wavefront's lanes
 }
 
 size_t rec_stride_of(const clifft_svm_program* p) {
-    size_t r = align_up(std::max<size_t>(p->total_slots + p->num_det + p->num_obs, 4), 16);
-    return r + (p->nw == 0 ? 256 : 0);
+    return align_up(std::max<size_t>(p->total_slots + p->num_det + p->num_obs, 4), 16);
 }
 
 // Choose the team shape and storage tier for a program's peak rank.

@@ -45,42 +45,45 @@ struct FrameReg {
     }
 };
 
-// Pauli frame of up to 16 words in this wavefront's LDS block (wider
-// programs).  Lane 0 writes, every lane reads after a wavefront fence.
-struct FrameLds {
-    __attribute__((address_space(3))) uint64_t* m;  // x[16] then z[16]
-    CSVM_INLINE static void wfence() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Pauli frame of up to 64 words (4096 qubits) distributed over the
+// wavefront: lane w holds word w of p_x and p_z in vector registers.  A
+// uniform word is read with v_readlane (uniform lane index); a word update is
+// a masked XOR in the owning lane.  Every wavefront of a team keeps its own
+// identical copy, so no memory or barrier is involved.
+struct FrameLane {
+    uint64_t x, z;
+    CSVM_INLINE static uint32_t my_lane() { return threadIdx.x & 63u; }
+    CSVM_INLINE static uint64_t readlane64(uint64_t v, uint32_t w) {
+        uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, w);
+        uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), w);
+        return ((uint64_t)hi << 32) | lo;
     }
-    CSVM_INLINE void init(void* base) {
-        m = (__attribute__((address_space(3))) uint64_t*)(base);
-        if ((threadIdx.x & 63u) == 0)
-            for (int i = 0; i < 32; ++i) m[i] = 0;
-        wfence();
-    }
-    CSVM_INLINE uint64_t wx(uint32_t w) const { return uni64(m[w]); }
-    CSVM_INLINE uint64_t wz(uint32_t w) const { return uni64(m[16 + w]); }
+    CSVM_INLINE void init(void*) { x = z = 0; }
+    CSVM_INLINE uint64_t wx(uint32_t w) const { return readlane64(x, w); }
+    CSVM_INLINE uint64_t wz(uint32_t w) const { return readlane64(z, w); }
     CSVM_INLINE uint32_t gx(uint32_t q) const { return (uint32_t)(wx(q >> 6) >> (q & 63)) & 1u; }
     CSVM_INLINE uint32_t gz(uint32_t q) const { return (uint32_t)(wz(q >> 6) >> (q & 63)) & 1u; }
     CSVM_INLINE void xor_words(uint32_t w, uint64_t ex, uint64_t ez) {
-        if ((threadIdx.x & 63u) == 0) {
-            m[w] ^= ex;
-            m[16 + w] ^= ez;
-        }
-        wfence();
+        bool mine = my_lane() == w;
+        x ^= mine ? ex : 0ull;
+        z ^= mine ? ez : 0ull;
     }
     CSVM_INLINE void fx(uint32_t q, uint32_t b) { xor_words(q >> 6, (uint64_t)(b & 1u) << (q & 63), 0); }
     CSVM_INLINE void fz(uint32_t q, uint32_t b) { xor_words(q >> 6, 0, (uint64_t)(b & 1u) << (q & 63)); }
     CSVM_INLINE void set_words(uint32_t w, uint64_t vx, uint64_t vz) {
-        if ((threadIdx.x & 63u) == 0) {
-            m[w] = vx;
-            m[16 + w] = vz;
-        }
-        wfence();
+        bool mine = my_lane() == w;
+        x = mine ? vx : x;
+        z = mine ? vz : z;
     }
 };
+
+// Record-byte fence: lane 0 of a wavefront writes a record byte that every
+// lane then reads back.
+CSVM_INLINE void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <class F, bool BT, class AmpT>
 struct Interp {
@@ -155,7 +158,7 @@ struct Interp {
     // ------------------------------------------------------------ records
     CSVM_INLINE void rec_write(__attribute__((address_space(3))) uint8_t* r, uint32_t idx, uint32_t val) {
         if ((threadIdx.x & 63u) == 0) r[idx] = (uint8_t)val;
-        FrameLds::wfence();
+        wave_fence();
     }
     CSVM_INLINE uint32_t rec_read(const __attribute__((address_space(3))) uint8_t* r, uint32_t idx) {
         return uni(r[idx]);
@@ -261,7 +264,7 @@ struct Interp {
     CSVM_INLINE bool choose(double p0, double p1, double total, uint32_t frame_bit, uint32_t sign, uint32_t idx,
                             bool is_forced, uint32_t& b, uint32_t& m_abs, uint32_t& rec) {
         if (is_forced) {
-            rec = uni(forced[idx]);
+            rec = uni(load_u8(forced, idx));
             m_abs = rec ^ sign;
             b = m_abs ^ frame_bit;
             double inc;
@@ -336,7 +339,7 @@ struct Interp {
         uint32_t W = P->words;
         cptr<uint64_t> mx = cp(P->ev_x) + (size_t)ci * W;
         cptr<uint64_t> mz = cp(P->ev_z) + (size_t)ci * W;
-        uint32_t fsign = cp(P->ev_sign)[ci] ? 1u : 0u;
+        uint32_t fsign = load_u8(cp(P->ev_sign), ci) ? 1u : 0u;
         uint32_t parity = 0;
         for (uint32_t w = 0; w < W; ++w) parity += __popcll(mx[w] & f.wz(w)) + __popcll(mz[w] & f.wx(w));
         if (parity & 1u) fsign ^= 1u;
@@ -451,7 +454,7 @@ struct Interp {
                     cptr<clifft_u2_node> node = cp(P->u2) + i0;
                     uint32_t s = (f.gz(a1) << 1) | f.gx(a1);
                     mul_phase(node->gamma[s][0], node->gamma[s][1]);
-                    uint32_t out = node->out_states[s];
+                    uint32_t out = load_u8(&node->out_states[0], s);
                     set_x(a1, out & 1u);
                     set_z(a1, (out >> 1) & 1u);
                     if (a1 < k) sweep_u2<BT>(amp, k, a1, &node->matrices[s][0][0], g);
@@ -462,7 +465,7 @@ struct Interp {
                     uint32_t s = (f.gz(a2) << 3) | (f.gx(a2) << 2) | (f.gz(a1) << 1) | f.gx(a1);
                     cptr<clifft_u4_entry> e = &cp(P->u4)[i0].entries[s];
                     mul_phase(e->gamma[0], e->gamma[1]);
-                    uint32_t out = e->out_state;
+                    uint32_t out = load_u8(&e->out_state, 0);
                     set_x(a1, out & 1u);
                     set_z(a1, (out >> 1) & 1u);
                     set_x(a2, (out >> 2) & 1u);
@@ -503,7 +506,7 @@ struct Interp {
                 case CLIFFT_OP_SWAP_MEAS_INTERFERE: swap_meas_interfere(a1, a2, i0, sign, false); break;
                 case CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED: {  // svm_forced_kernels.cc:63-89
                     uint32_t det_v = (flags & CLIFFT_FLAG_IDENTITY) ? sign : (f.gx(a1) ^ sign);
-                    if (uni(forced[i0]) != det_v) {
+                    if (uni(load_u8(forced, i0)) != det_v) {
                         reachable = 0;
                         return;
                     }
@@ -511,7 +514,7 @@ struct Interp {
                     break;
                 }
                 case CLIFFT_OP_MEAS_DORMANT_RANDOM_FORCED: {  // svm_forced_kernels.cc:91-116
-                    uint32_t req = uni(forced[i0]);
+                    uint32_t req = uni(load_u8(forced, i0));
                     uint32_t m = req ^ sign;
                     logp += log(0.5);
                     if (f.gx(a1) && m) negate_gamma();
@@ -533,7 +536,7 @@ struct Interp {
                     if (rec_read(meas, i1)) {
                         cptr<DevProgram> P = opaque(Pg);
                         uint32_t W = P->words;
-                        apply_pauli(cp(P->pauli_x) + (size_t)i0 * W, cp(P->pauli_z) + (size_t)i0 * W, W, cp(P->pauli_sign)[i0]);
+                        apply_pauli(cp(P->pauli_x) + (size_t)i0 * W, cp(P->pauli_z) + (size_t)i0 * W, W, load_u8(cp(P->pauli_sign), i0));
                     }
                     break;
                 case CLIFFT_OP_NOISE:

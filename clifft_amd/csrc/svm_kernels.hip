@@ -4,9 +4,9 @@
 
 namespace csvm {
 
-// Frame kinds: one word in scalar registers (<= 64 qubits), or up to 16
-// words in LDS.
-enum : int { FRAME_W1 = 1, FRAME_LDS = 0 };
+// Frame kinds: one word in scalar registers (<= 64 qubits), or up to 64
+// words spread over the wavefront's lanes (<= 4096 qubits).
+enum : int { FRAME_W1 = 1, FRAME_LANES = 0 };
 
 template <int FK>
 struct FrameOf;
@@ -15,15 +15,14 @@ struct FrameOf<FRAME_W1> {
     typedef FrameReg<1> type;
 };
 template <>
-struct FrameOf<FRAME_LDS> {
-    typedef FrameLds type;
+struct FrameOf<FRAME_LANES> {
+    typedef FrameLane type;
 };
 
 // LDS carve-up of one workgroup:
 //   [teams_per_block x amp_elems x 16 B]  amplitudes (LDS tier only)
 //   [512 B]                               team reduction scratch (16 x 4 f64)
-//   [waves_per_block x rec_stride]        per-wavefront block: frame (LDS
-//                                         frame kind: 256 B) then records
+//   [waves_per_block x rec_stride]        per-wavefront record copies
 template <int FK, bool BT, bool GA>
 __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag) {
     typedef typename FrameOf<FK>::type F;
@@ -58,7 +57,7 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
     S.red = (lds_f64*)(p);
     p += 16 * 4 * sizeof(double);
     unsigned char* wblock = p + (size_t)wave_in_block * rec_stride;
-    unsigned char* rec = wblock + (FK == FRAME_LDS ? 256 : 0);
+    unsigned char* rec = wblock;
     typedef __attribute__((address_space(3))) uint8_t lds_u8;
     {
         cptr<DevProgram> P = opaque(Pg);
@@ -81,7 +80,7 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
         const uint32_t mode = A->mode;
         // ---- per-shot reset (svm_state.cc:220-267) ----
         for (uint32_t i = wl; i < rec_stride / 4; i += 64) reinterpret_cast<uint32_t*>(wblock)[i] = 0u;
-        FrameLds::wfence();
+        wave_fence();
         S.f.init(wblock);
         S.dust = 0;
         S.discarded = 0;
@@ -104,8 +103,8 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
             uint32_t n = 1u << P->peak_rank;
             for (uint32_t i = S.g.lane; i < n; i += S.g.size) S.amp[i] = cp(A->preset_amps)[i];
             uint32_t nrec = P->total_meas_slots + P->num_det + P->num_obs;
-            for (uint32_t i = wl; i < nrec; i += 64) rec[i] = cp(A->preset_records)[i];
-            FrameLds::wfence();
+            for (uint32_t i = wl; i < nrec; i += 64) rec[i] = (uint8_t)load_u8(cp(A->preset_records), i);
+            wave_fence();
             S.forced = cp(A->forced_records);
         } else {
             S.g_re = 1.0;
@@ -180,7 +179,7 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
                                 uint32_t i = 8 * j + b;
                                 if (i < n) {
                                     uint32_t v = src[i];
-                                    if (xor_expected && i < n_exp) v ^= expo[i] ? 1u : 0u;
+                                    if (xor_expected && i < n_exp) v ^= load_u8(expo, i) ? 1u : 0u;
                                     byte |= (v & 1u) << b;
                                 }
                             }
@@ -189,7 +188,7 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
                     } else {
                         for (uint32_t i = wl; i < n; i += 64) {
                             uint32_t v = src[i];
-                            if (xor_expected && i < n_exp) v ^= expo[i] ? 1u : 0u;
+                            if (xor_expected && i < n_exp) v ^= load_u8(expo, i) ? 1u : 0u;
                             out[s * n + i] = (uint8_t)v;
                         }
                     }
@@ -240,7 +239,7 @@ static hipError_t launch_fk(const LaunchShape& L, const DevProgram* P, const Sho
 hipError_t launch_shots(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
     switch (L.nw) {
         case 1: return launch_fk<FRAME_W1>(L, P, A, st);
-        default: return launch_fk<FRAME_LDS>(L, P, A, st);
+        default: return launch_fk<FRAME_LANES>(L, P, A, st);
     }
 }
 
