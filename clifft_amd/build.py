@@ -1,6 +1,6 @@
 """Build recipe for the native libraries (in-tree, gfx950 only).
 
-    python -m clifft_amd.build          # builds clifft_amd/libclifft_svm.so
+    python -m clifft_amd.build   # clifft_amd/libclifft_svm.so + libclifft_compile.so
 
 The library is compiled with hipcc for gfx950 with packed-FP32 instruction
 selection disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -49,6 +49,8 @@ def build_svm(verbose: bool = True) -> str:
 
 def build_all() -> None:
     build_svm()
+    from clifft_amd import compile as _compile
+    _compile.build()
 
 
 if __name__ == "__main__":

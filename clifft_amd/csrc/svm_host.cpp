@@ -355,9 +355,16 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots) {
     uint32_t per_cu = std::min(by_lds, by_waves);
     uint64_t cap = (uint64_t)di.num_cus * per_cu;
     if (L.global_amps) {
-        // bound the global scratch to 8 GiB
+        // bound the global scratch by what the device has free (keep 40% or
+        // 16 GiB headroom, whichever is larger), at most 192 GiB
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 8ull << 30;
+        size_t budget = free_b + p->gscratch_bytes;
+        size_t headroom = std::max<size_t>(budget * 2 / 5, 16ull << 30);
+        budget = budget > headroom ? budget - headroom : (size_t)1 << 30;
+        budget = std::min<size_t>(budget, 192ull << 30);
         size_t per_team = (size_t)16 << p->peak_rank;
-        uint64_t max_teams = std::max<uint64_t>(1, (8ull << 30) / per_team);
+        uint64_t max_teams = std::max<uint64_t>(1, budget / per_team);
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, max_teams / tpb));
     }
     L.grid = (uint32_t)std::max<uint64_t>(1, std::min(blocks_needed, cap));

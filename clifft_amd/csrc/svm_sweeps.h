@@ -36,6 +36,14 @@ struct TeamGeom {
     uint32_t nwaves;  // wavefronts in team
 };
 
+// Global-memory sweeps keep several independent 16-byte loads in flight per
+// lane (loads of a whole group are issued before its stores); LDS sweeps do
+// one element per trip.
+template <class AmpT>
+struct Unroll {
+    static constexpr uint32_t value = std::is_same<AmpT, glb_d2>::value ? 4u : 1u;
+};
+
 // Team sum of N doubles, identical bits on every lane: lane butterfly, then
 // (workgroup teams) waves summed in index order through LDS scratch `red`
 // (16 x 4 doubles), fenced by a barrier on both sides.
@@ -144,11 +152,27 @@ template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_h(AmpT* amp, uint32_t k, uint32_t v, TeamGeom g) {
     k = uni(k); v = uni(v); g = uni_geom(g);
     uint32_t half = 1u << (k - 1), vb = 1u << v;
-    for (uint32_t i = g.lane; i < half; i += g.size) {
-        uint32_t i0 = insert_zero_bit(i, v), i1 = i0 | vb;
-        dv2 a = amp[i0], b = amp[i1];
-        amp[i0] = cscale(cadd(a, b), kInvSqrt2);
-        amp[i1] = cscale(csub(a, b), kInvSqrt2);
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    const uint32_t step = U * g.size;
+    for (uint32_t j0 = g.lane; j0 < half; j0 += step) {
+        dv2 a[U], b[U];
+        uint32_t i0[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            uint32_t i = j0 + u * g.size;
+            i0[u] = insert_zero_bit(i, v);
+            if (i < half) {
+                a[u] = amp[i0[u]];
+                b[u] = amp[i0[u] | vb];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (j0 + u * g.size < half) {
+                amp[i0[u]] = cscale(cadd(a[u], b[u]), kInvSqrt2);
+                amp[i0[u] | vb] = cscale(csub(a[u], b[u]), kInvSqrt2);
+            }
+        }
     }
     team_sync<BT>();
 }
@@ -158,9 +182,20 @@ CSVM_SWEEP void sweep_phase(AmpT* amp, uint32_t k, uint32_t v, double pr, double
     k = uni(k); v = uni(v); pr = unid(pr); pi = unid(pi); g = uni_geom(g);
     uint32_t half = 1u << (k - 1), vb = 1u << v;
     dv2 ph = mk(pr, pi);
-    for (uint32_t i = g.lane; i < half; i += g.size) {
-        uint32_t idx = insert_zero_bit(i, v) | vb;
-        amp[idx] = cmul(amp[idx], ph);
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    const uint32_t step = U * g.size;
+    for (uint32_t i0 = g.lane; i0 < half; i0 += step) {
+        dv2 x[U];
+        uint32_t idx[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            uint32_t i = i0 + u * g.size;
+            idx[u] = insert_zero_bit(i, v) | vb;
+            if (i < half) x[u] = amp[idx[u]];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < half) amp[idx[u]] = cmul(x[u], ph);
     }
     team_sync<BT>();
 }
@@ -172,11 +207,27 @@ CSVM_SWEEP void sweep_u2(AmpT* amp, uint32_t k, uint32_t v, cptr<double> m, Team
     dv2 m00 = mk(m[0], m[1]), m01 = mk(m[2], m[3]);
     dv2 m10 = mk(m[4], m[5]), m11 = mk(m[6], m[7]);
     uint32_t half = 1u << (k - 1), vb = 1u << v;
-    for (uint32_t i = g.lane; i < half; i += g.size) {
-        uint32_t i0 = insert_zero_bit(i, v), i1 = i0 | vb;
-        dv2 a = amp[i0], b = amp[i1];
-        amp[i0] = cadd(cmul(m00, a), cmul(m01, b));
-        amp[i1] = cadd(cmul(m10, a), cmul(m11, b));
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    const uint32_t step = U * g.size;
+    for (uint32_t j0 = g.lane; j0 < half; j0 += step) {
+        dv2 a[U], b[U];
+        uint32_t i0[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            uint32_t i = j0 + u * g.size;
+            i0[u] = insert_zero_bit(i, v);
+            if (i < half) {
+                a[u] = amp[i0[u]];
+                b[u] = amp[i0[u] | vb];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (j0 + u * g.size < half) {
+                amp[i0[u]] = cadd(cmul(m00, a[u]), cmul(m01, b[u]));
+                amp[i0[u] | vb] = cadd(cmul(m10, a[u]), cmul(m11, b[u]));
+            }
+        }
     }
     team_sync<BT>();
 }
@@ -215,11 +266,21 @@ template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_expand(AmpT* amp, uint32_t k, double pr, double pi, uint32_t has_phase, TeamGeom g) {
     k = uni(k); pr = unid(pr); pi = unid(pi); has_phase = uni(has_phase); g = uni_geom(g);
     uint32_t half = 1u << k;
-    dv2 ph = mk(pr, pi);
-    if (has_phase) {
-        for (uint32_t i = g.lane; i < half; i += g.size) amp[i + half] = cmul(amp[i], ph);
-    } else {
-        for (uint32_t i = g.lane; i < half; i += g.size) amp[i + half] = amp[i];
+    dv2 ph = has_phase ? mk(pr, pi) : mk(1.0, 0.0);
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    const uint32_t step = U * g.size;
+    for (uint32_t i0 = g.lane; i0 < half; i0 += step) {
+        dv2 v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            uint32_t i = i0 + u * g.size;
+            if (i < half) v[u] = amp[i];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            uint32_t i = i0 + u * g.size;
+            if (i < half) amp[i + half] = has_phase ? cmul(v[u], ph) : v[u];
+        }
     }
     team_sync<BT>();
 }
@@ -228,7 +289,16 @@ template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_scale(AmpT* amp, uint32_t k, double s, TeamGeom g) {
     k = uni(k); s = unid(s); g = uni_geom(g);
     uint32_t n = 1u << k;
-    for (uint32_t i = g.lane; i < n; i += g.size) amp[i] = cscale(amp[i], s);
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    for (uint32_t i0 = g.lane; i0 < n; i0 += U * g.size) {
+        dv2 x[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < n) x[u] = amp[i0 + u * g.size];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < n) amp[i0 + u * g.size] = cscale(x[u], s);
+    }
     team_sync<BT>();
 }
 
@@ -276,7 +346,16 @@ template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_take_upper(AmpT* amp, uint32_t k, TeamGeom g) {
     k = uni(k); g = uni_geom(g);
     uint32_t half = 1u << (k - 1);
-    for (uint32_t i = g.lane; i < half; i += g.size) amp[i] = amp[i + half];
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    for (uint32_t i0 = g.lane; i0 < half; i0 += U * g.size) {
+        dv2 x[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < half) x[u] = amp[i0 + u * g.size + half];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < half) amp[i0 + u * g.size] = x[u];
+    }
     team_sync<BT>();
 }
 
@@ -285,9 +364,22 @@ template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_fold(AmpT* amp, uint32_t k, double s, TeamGeom g) {
     k = uni(k); s = unid(s); g = uni_geom(g);
     uint32_t half = 1u << (k - 1);
-    for (uint32_t i = g.lane; i < half; i += g.size) {
-        dv2 a = amp[i], b = amp[i + half];
-        amp[i] = mk((a.x + s * b.x) * kInvSqrt2, (a.y + s * b.y) * kInvSqrt2);
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    for (uint32_t i0 = g.lane; i0 < half; i0 += U * g.size) {
+        dv2 a[U], b[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            uint32_t i = i0 + u * g.size;
+            if (i < half) {
+                a[u] = amp[i];
+                b[u] = amp[i + half];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            uint32_t i = i0 + u * g.size;
+            if (i < half) amp[i] = mk((a[u].x + s * b[u].x) * kInvSqrt2, (a[u].y + s * b[u].y) * kInvSqrt2);
+        }
     }
     team_sync<BT>();
 }

@@ -75,9 +75,9 @@ ARRAY_OPS = frozenset(
 # --------------------------------------------------------------------------
 INSTR_DTYPE = np.dtype({
     "names": ["opcode", "reserved", "flags", "pad", "axis_1", "axis_2",
-              "weight_re", "weight_im", "idx0", "idx1", "mask"],
-    "formats": ["u1", "u1", "u1", "u1", "<u2", "<u2", "<f8", "<f8", "<u4", "<u4", "<u8"],
-    "offsets": [0, 1, 2, 3, 4, 6, 8, 16, 8, 12, 8],
+              "weight_re", "weight_im", "idx0", "idx1", "mask", "tail"],
+    "formats": ["u1", "u1", "u1", "u1", "<u2", "<u2", "<f8", "<f8", "<u4", "<u4", "<u8", "<u8"],
+    "offsets": [0, 1, 2, 3, 4, 6, 8, 16, 8, 12, 8, 24],
     "itemsize": 32,
 })
 
@@ -310,8 +310,10 @@ class Program:
     u4_nodes: np.ndarray = None
     expected_observables: np.ndarray = None
     global_weight: complex = 1.0 + 0.0j
-    # parameter slots: instruction index of each parameterised rotation
+    # parameter slots: instruction index of each parameterised rotation and
+    # the sign s of its weight z = exp(i pi s alpha)
     param_instr: np.ndarray = None
+    param_sign: np.ndarray = None
     _view: ProgramView = field(default=None, repr=False, compare=False)
 
     def __post_init__(self):
@@ -352,6 +354,7 @@ class Program:
             self.u4_nodes = np.zeros(0, dtype=U4_DTYPE)
         z("expected_observables", np.uint8)
         z("param_instr", np.uint32)
+        z("param_sign", np.int8)
         self.pauli_x = self.pauli_x.reshape(-1)
         self.pauli_z = self.pauli_z.reshape(-1)
         self.ev_x = self.ev_x.reshape(-1)
@@ -448,7 +451,7 @@ class Program:
     _ARRAYS = ["bytecode", "pauli_x", "pauli_z", "pauli_sign", "ev_x", "ev_z", "ev_sign",
                "site_chan_off", "chan_x", "chan_z", "chan_prob", "hazards", "ro_meas_idx",
                "ro_prob", "det_off", "det_tgt", "obs_off", "obs_tgt", "u2_nodes", "u4_nodes",
-               "expected_observables", "param_instr"]
+               "expected_observables", "param_instr", "param_sign"]
     _SCALARS = ["num_qubits", "peak_rank", "num_measurements", "total_meas_slots",
                 "num_detectors", "num_observables", "num_exp_vals", "has_postselection"]
 
@@ -478,7 +481,7 @@ class Program:
                   "pauli_x": np.uint64, "pauli_z": np.uint64, "ev_x": np.uint64, "ev_z": np.uint64,
                   "chan_x": np.uint64, "chan_z": np.uint64, "pauli_sign": np.uint8,
                   "ev_sign": np.uint8, "expected_observables": np.uint8, "chan_prob": np.float64,
-                  "hazards": np.float64, "ro_prob": np.float64}
+                  "hazards": np.float64, "ro_prob": np.float64, "param_sign": np.int8}
         arrays = {}
         for name in cls._ARRAYS:
             (n,) = struct.unpack_from("<Q", blob, pos)

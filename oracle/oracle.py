@@ -216,11 +216,12 @@ def execute_forced_faults(prog, state: State, noise_idx, readout_idx):
     state.L.orc_execute_forced_faults(ctypes.byref(v), state.h, _p(n), len(n), _p(r), len(r))
 
 
-def sample(prog, shots, seed=0, per_shot=True, first_shot=0):
+def sample(prog, shots, seed=0, per_shot=True, first_shot=0, fast=False):
     """Oracle sample().  per_shot=False reproduces the reference single
     stream (src/clifft/svm/svm.cc:191-255); per_shot=True is the device
-    executor's per-shot stream convention."""
-    L = lib()
+    executor's per-shot stream convention.  fast=True uses the -O3
+    -ffast-math build (CPU-baseline timing only)."""
+    L = lib(fast)
     v = prog.view()
     m = np.zeros((shots, prog.num_measurements), np.uint8)
     d = np.zeros((shots, prog.num_detectors), np.uint8)

@@ -14,22 +14,32 @@ from clifft_amd.program import INSTR_DTYPE, U2_DTYPE, U4_DTYPE, ProgramView
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
-        text = open(h).read()
-        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names |= set(re.findall(r"\b(clifft_\w+)\s*\(", text))
-    return names
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(clifft_\w+)\s*\(", text))
 
 
-def test_library_exports_every_declared_symbol():
+def test_every_header_is_covered():
+    headers = {os.path.basename(h) for h in glob.glob(os.path.join(ROOT, "include", "*.h"))}
+    assert headers == {"clifft_svm.h", "clifft_compile.h"}
+
+
+def test_svm_library_exports_every_declared_symbol():
     lib = L.lib()
-    decl = declared_functions()
+    decl = declared_functions("clifft_svm.h")
     assert decl, "no declarations found"
     missing = [n for n in sorted(decl) if not hasattr(lib, n)]
     assert not missing, missing
     assert decl == set(L.SIGNATURES), "ctypes signature table out of sync with the header"
+
+
+def test_compile_library_exports_every_declared_symbol():
+    from clifft_amd import compile as C
+    lib = C.lib()
+    decl = declared_functions("clifft_compile.h")
+    missing = [n for n in sorted(decl) if not hasattr(lib, n)]
+    assert decl and not missing, missing
 
 
 def test_abi_version_and_layouts():
