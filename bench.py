@@ -10,9 +10,13 @@ surface-code memory circuit (1e6 shots per step).  Prints one JSON line.
 roofline: algorithmic bytes of one launch (per-shot state-traffic model:
 16 B per amplitude read or written by each array opcode at its static rank,
 plus the record bytes written) / the shot kernel's average duration from HIP
-events on its stream, against the 8 TB/s HBM peak.
-cpu_baseline: the C oracle built with -O3 -ffast-math, single PRNG stream,
-shots in sequence (the reference's scheduling), on a bounded sample.
+events on its stream, against the 8 TB/s HBM peak.  `traffic` is the HBM
+bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters
+(profiles/hbm_traffic.json, scaled by a copy-kernel calibration), or null
+when no measurement exists for the workload at this shot count.
+cpu_baseline: the C oracle built with -O3 -ffast-math -fopenmp, single PRNG
+stream, shots in sequence (the reference's scheduling), on a bounded sample;
+the shot-parallel OpenMP figure on all host cores rides along as "mode_b".
 """
 
 from __future__ import annotations
@@ -22,6 +26,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -33,6 +39,8 @@ WORKLOADS = {
     "surface_d3": ("noisy d=3 rotated surface-code memory, 3 rounds, p=1e-3, 1 T/round (17 qubits)", 1_000_000),
     "clifford_t20": ("20 qubits x 40 layers random Clifford+T, 6 T, final M (peak k=6)", 100_000),
     "large_k34": ("34 qubits x 40 layers, 26 T on distinct qubits, MR on 4 qubits / 10 layers", 256),
+    "batch4096": ("4096 random Clifford+T programs, n ~ U[16,40], T-count ~ U[4,16], 40 layers, final M; "
+                  "one persistent launch", 1000),
 }
 
 
@@ -48,21 +56,153 @@ def build_program(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(prog, target_seconds=12.0):
-    """Oracle (fast build) single stream, shots in sequence, on a bounded
-    sample sized to about `target_seconds`."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _timed(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(prog, run_seconds=2.5, max_shots=20_000_000):
+    """The C oracle built like the reference Release build (-O3 -ffast-math
+    -fopenmp), timed on this host on a bounded sample of the same program:
+    mode A = one PRNG stream, shots in sequence (the reference's scheduling,
+    src/clifft/svm/svm.cc:191-255), median of 5 runs after a warm-up;
+    mode B = OpenMP over shots with per-shot seeds on every host core,
+    median of 3.  About 20 s of CPU time in all."""
     from oracle import oracle
-    shots = 1000
+    probe = 100 if prog.peak_rank < 20 else 1
+    while True:
+        t0 = time.perf_counter()
+        oracle.sample(prog, probe, seed=7, per_shot=False, fast=True)
+        dt = time.perf_counter() - t0
+        if dt > 0.25 or prog.peak_rank >= 20:
+            break
+        probe *= 4
+    per_shot = max(dt / probe, 1e-9)
+    n_a = int(min(max(run_seconds / per_shot, 1), max_shots))
+    oracle.sample(prog, min(n_a, 1000), seed=7, per_shot=False, fast=True)
+    t_a = _timed(lambda: oracle.sample(prog, n_a, seed=7, per_shot=False, fast=True), 5)
+    threads = oracle.lib(True).orc_max_threads()
+    n_b = int(min(max(run_seconds / per_shot * threads * 0.6, threads), max_shots))
+    mem_b = n_b * (prog.num_measurements + prog.num_detectors + prog.num_observables + 8 * prog.num_exp_vals)
+    if mem_b > (8 << 30):
+        n_b = max(threads, int(n_b * (8 << 30) / mem_b))
+    t_b = _timed(lambda: oracle.sample_parallel(prog, n_b, seed=7, threads=threads, fast=True), 3)
+    host = f"{os.cpu_count()} logical CPUs, {_cpu_model()}"
+    return {"value": n_a / t_a, "unit": "shots/s", "cores": 1, "kind": "port",
+            "sample": f"mode A: {n_a} shots of the same program, one stream, median of 5 = {t_a:.2f} s; "
+                      f"host {host}",
+            "mode_b": {"value": n_b / t_b, "unit": "shots/s", "cores": threads,
+                       "sample": f"{n_b} shots, OpenMP over shots, per-shot seeds, median of 3 = {t_b:.2f} s"}}
+
+
+def run_batch(args):
+    """Config: 4096 programs x 1000 shots in one persistent launch (autotuned),
+    against one plain launch per program on the same stream."""
+    import torch
+    from clifft_amd import _lib, svm
+    from clifft_amd import batch as B
+    from clifft_amd import workloads as W
+    from clifft_amd.compile import compile as ccompile
+
+    desc, shots = WORKLOADS["batch4096"]
+    count = int(os.environ.get("CLIFFT_BENCH_BATCH_COUNT", "4096"))
+    if args.shots:
+        shots = args.shots
+    progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
+    seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
+    dev = torch.device("cuda:0")
+    nb = lambda n: (n + 7) // 8
+    bufs, outs = [], []
+    for p in progs:
+        m = torch.empty((shots, nb(p.num_measurements)), dtype=torch.uint8, device=dev)
+        bufs.append(m)
+        outs.append({"measurements": m.data_ptr()})
+    handles = [svm.device_program(p) for p in progs]
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    bt = B.Batch(progs, [shots] * count, outs, packed=True)
+    tuning = bt.tune(sample_shots=max(1, shots // 8), stream=sh)
+    torch.cuda.synchronize()
+
+    # per-program plain launches (the unbatched baseline), same outputs
+    recs = (_lib.Records * count)(*[_lib.Records(o["measurements"], None, None, None, None) for o in outs])
+    harr = (__import__("ctypes").c_void_p * count)(*[h.handle.value for h in handles])
+    shots_arr = np.full(count, shots, dtype=np.uint64)
+    flags_pp = _lib.REC_DEVICE | _lib.REC_PACKED | _lib.BATCH_PER_PROGRAM
+
+    def per_program(seeds):
+        _lib.check(_lib.lib().clifft_svm_batch_sample(harr, shots_arr.ctypes.data, seeds.ctypes.data, count,
+                                                      flags_pp, recs, sh), "per-program")
+
+    per_program(seeds0)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    oracle.sample(prog, shots, seed=7, per_shot=False, fast=True)
-    dt = time.perf_counter() - t0
-    est = max(100, int(shots * target_seconds / max(dt, 1e-6) * 0.8))
-    est = min(est, 20_000_000)
-    t0 = time.perf_counter()
-    oracle.sample(prog, est, seed=7, per_shot=False, fast=True)
-    dt = time.perf_counter() - t0
-    return {"value": est / dt, "unit": "shots/s", "cores": 1, "kind": "port",
-            "sample": f"{est} shots of the same program, oracle -O3 -ffast-math, one stream, {dt:.1f} s"}
+    reps_pp = max(1, min(3, args.steps))
+    for r in range(reps_pp):
+        per_program(seeds0 + np.uint64(r + 1))
+    torch.cuda.synchronize()
+    t_pp = (time.perf_counter() - t0) / reps_pp
+
+    for i in range(args.warmup):
+        bt.run(seeds0 + np.uint64(100 + i), stream=sh)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    with torch.cuda.stream(stream):
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            starts[i].record(stream)
+            bt.run(seeds0 + np.uint64(1000 + i), stream=sh)
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        bt.run_graph(seeds0, stream=sh)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            bt.run_graph(seeds0 + np.uint64(2000 + i), stream=sh)
+        torch.cuda.synchronize()
+        wall_graph = time.perf_counter() - t0
+    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    total = shots * count
+    value = total * args.steps / wall
+    alg = sum((h.info()["bytes_per_shot"] + nb(p.num_measurements)) * shots for h, p in zip(handles, progs))
+    achieved = alg / (kern_ms * 1e-3) / 1e9
+    peaks = np.bincount([p.peak_rank for p in progs], minlength=17)
+    line = {
+        "metric": "shots/s", "value": value, "unit": "shots/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batch4096", "description": desc, "programs": count, "shots_per_program": shots,
+                   "peak_rank_histogram": peaks.tolist(),
+                   "mean_instructions": float(np.mean([len(p.bytecode) for p in progs])),
+                   "tuning": tuning, "records": "bit-packed measurements, device memory",
+                   "per_program_launches_shots_per_s": total / t_pp,
+                   "speedup_vs_per_program": (total / (wall / args.steps)) / (total / t_pp),
+                   "graph_replay_shots_per_s": total * args.steps / wall_graph,
+                   "parallelism": "programs x shots"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kern_ms,
+                     "algorithmic_bytes_per_launch": alg},
+        "backend": _lib.backend(),
+    }
+    bt.close()
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -73,18 +213,30 @@ def main():
     ap.add_argument("--workload", default="surface_d3", choices=sorted(WORKLOADS))
     ap.add_argument("--shots", type=int, default=0, help="override shots per step")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this executor runs on one GPU; multi-GPU sharding of shots is not "
-                                   "implemented (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this executor runs on one GPU; sharding shots over several GPUs is "
+                                   "not implemented", "n_gpus": args.gpus}))
         raise SystemExit(2)
 
-    import numpy as np
     import torch
     from clifft_amd import _lib, svm
 
     assert torch.cuda.is_available(), "needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibrate:
+        n = args.copy_calibrate << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": args.steps}), flush=True)
+        return
+    if args.workload == "batch4096":
+        return run_batch(args)
     desc, shots = WORKLOADS[args.workload]
     if args.shots:
         shots = args.shots
@@ -122,6 +274,14 @@ def main():
     kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     value = shots * args.steps / wall
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        w = tr["workloads"].get(args.workload)
+        if w and w["shots"] == shots:
+            traffic = (w["fetch_kib"] * tr["fetch_scale"] + w["write_kib"] * tr["write_scale"]) * 1024.0
     rec_bytes = nb(prog.num_measurements) + nb(prog.num_detectors) + nb(prog.num_observables)
     bytes_per_launch = (info["bytes_per_shot"] + rec_bytes) * shots
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
@@ -136,7 +296,7 @@ def main():
                    "shots_per_block": info["shots_per_block"], "records": "bit-packed, device memory",
                    "parallelism": "shots"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
         "backend": _lib.backend(),
     }

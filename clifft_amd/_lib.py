@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libclifft_svm.so")
 
 REC_DEVICE = 1
 REC_PACKED = 2
+BATCH_PER_PROGRAM = 0x100
 
 E_INVALID, E_UNSUPPORTED, E_HIP, E_NOMEM, E_NODEVICE = -1, -2, -3, -4, -5
 
@@ -52,6 +53,12 @@ class State(ctypes.Structure):
                 ("forced_record", ctypes.c_void_p), ("dust_clamps", ctypes.c_uint64)]
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("block_threads", ctypes.c_uint32), ("wave_rank_max", ctypes.c_uint32),
+                ("lds_rank_max", ctypes.c_uint32), ("wave_chunk", ctypes.c_uint32),
+                ("block_chunk", ctypes.c_uint32), ("grid", ctypes.c_uint32)]
+
+
 # name -> (restype, argtypes); exactly the functions include/clifft_svm.h declares
 SIGNATURES = {
     "clifft_svm_abi_version": (ctypes.c_uint32, []),
@@ -68,6 +75,16 @@ SIGNATURES = {
     "clifft_svm_record_log_probs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                                    ctypes.c_void_p, ctypes.c_void_p]),
     "clifft_svm_execute": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(State)]),
+    "clifft_svm_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_uint32,
+                                               ctypes.c_uint32, ctypes.POINTER(Records),
+                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "clifft_svm_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "clifft_svm_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchTuning)]),
+    "clifft_svm_batch_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchTuning)]),
+    "clifft_svm_batch_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                             ctypes.POINTER(BatchTuning)]),
+    "clifft_svm_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "clifft_svm_batch_run_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "clifft_svm_batch_sample": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                                ctypes.POINTER(Records), ctypes.c_void_p]),

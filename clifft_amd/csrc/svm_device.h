@@ -116,6 +116,28 @@ struct ShotArgs {
     unsigned long long* dust_counter; // optional telemetry
 };
 
+// Batched executor: one work-queue entry is a contiguous range of shots of
+// one program, run by one workgroup with the team shape named by `tier`.
+enum : uint32_t { TIER_WAVE = 0, TIER_LDS = 1, TIER_GLOBAL = 2 };
+
+struct BatchItem {
+    uint32_t prog;  // index into BatchArgs::progs / args / seeds
+    uint32_t tier;
+    uint32_t shot_lo, shot_hi;
+};
+
+struct BatchArgs {
+    const BatchItem* items;          // sorted by decreasing estimated cost
+    const DevProgram* const* progs;  // device copy of each program's DevProgram
+    const ShotArgs* args;            // per program (outputs, rec_stride, amp_elems)
+    const uint64_t* seeds;           // per program
+    uint32_t* counter;               // queue head, cleared by a fill kernel before each launch
+    dv2* gscratch;                   // TIER_GLOBAL amplitudes, gscratch_elems per workgroup
+    uint64_t gscratch_elems;
+    uint32_t n_items;
+    uint32_t pad;
+};
+
 // Read-only program tables are dereferenced through constant-address-space
 // pointers so uniform reads become scalar (s_load) instructions.
 template <class T>
@@ -125,9 +147,6 @@ CSVM_INLINE cptr<T> cp(const T* p) {
     return (cptr<T>)(p);
 }
 
-// Hide a table pointer's provenance from loop-invariant code motion so table
-// fields are loaded where they are used instead of pinning registers across
-// the interpreter loop.
 // Byte read from a constant-space table through the aligned dword that
 // contains it.  Scalar loads require a dword-aligned base address; forming
 // base + byte_index and reusing it as the base of a later scalar load would
@@ -144,6 +163,9 @@ CSVM_INLINE uint64_t uni64_(uint64_t x) {
     uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
+// Hide a table pointer's provenance from loop-invariant code motion so table
+// fields are loaded where they are used instead of pinning registers across
+// the interpreter loop.
 template <class T>
 CSVM_INLINE cptr<T> opaque(cptr<T> p) {
     uint64_t v = uni64_((uint64_t)p);

@@ -57,6 +57,30 @@ DeviceInfo& device_info() {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// The library's own stream, used when the caller passes none.
+hipStream_t library_stream() {
+    static hipStream_t st = nullptr;
+    static std::once_flag once;
+    std::call_once(once, [] { (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking); });
+    return st;
+}
+
+// Global-memory amplitude scratch, one arena per stream: launches on one
+// stream run in order, so every program launched there can share it.
+struct Arena {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_arena_mu;
+std::vector<std::pair<hipStream_t, Arena>> g_arenas;
+
+Arena& arena_of(hipStream_t st) {
+    for (auto& a : g_arenas)
+        if (a.first == st) return a.second;
+    g_arenas.push_back({st, Arena{}});
+    return g_arenas.back().second;
+}
+
 // ---------------------------------------------------------------------------
 // Static validation: walk the bytecode with the static active-rank
 // trajectory and check every operand the kernels index with.
@@ -252,7 +276,6 @@ double op_bytes(const clifft_instruction& in, int32_t k, uint32_t fbits) {
 // Program object
 // ---------------------------------------------------------------------------
 struct clifft_svm_program {
-    hipStream_t stream = nullptr;
     void* dev_blob = nullptr;
     DevProgram dp{};                          // host copy (device pointers)
     DevProgram* dp_dev = nullptr;             // device copy, sampling bytecode
@@ -269,9 +292,7 @@ struct clifft_svm_program {
     double bytes_per_shot = 0.0;
     // launch planning
     LaunchShape base_shape;
-    // scratch (grown on demand)
-    void* gscratch = nullptr;
-    size_t gscratch_bytes = 0;
+    // staging for host outputs (grown on demand)
     void* out_dev = nullptr;
     size_t out_dev_bytes = 0;
     unsigned long long* dust_dev = nullptr;
@@ -324,14 +345,28 @@ LaunchShape plan_shape(const clifft_svm_program* p) {
     return L;
 }
 
-int ensure_scratch(clifft_svm_program* p, size_t bytes) {
-    if (bytes <= p->gscratch_bytes) return CLIFFT_OK;
-    if (p->gscratch) (void)hipFree(p->gscratch);
-    p->gscratch = nullptr;
-    p->gscratch_bytes = 0;
-    HIP_TRY(hipMalloc(&p->gscratch, bytes));
-    p->gscratch_bytes = bytes;
+// Grow the stream's arena to `bytes` (the old buffer may still be read by
+// queued launches, so wait for the stream before freeing it).
+int ensure_scratch(hipStream_t st, size_t bytes, void** ptr) {
+    std::lock_guard<std::mutex> lock(g_arena_mu);
+    Arena& a = arena_of(st);
+    if (bytes > a.bytes) {
+        if (a.ptr) {
+            HIP_TRY(hipStreamSynchronize(st));
+            (void)hipFree(a.ptr);
+        }
+        a.ptr = nullptr;
+        a.bytes = 0;
+        HIP_TRY(hipMalloc(&a.ptr, bytes));
+        a.bytes = bytes;
+    }
+    *ptr = a.ptr;
     return CLIFFT_OK;
+}
+
+size_t arena_bytes(hipStream_t st) {
+    std::lock_guard<std::mutex> lock(g_arena_mu);
+    return arena_of(st).bytes;
 }
 
 int ensure_out(clifft_svm_program* p, size_t bytes) {
@@ -345,7 +380,7 @@ int ensure_out(clifft_svm_program* p, size_t bytes) {
 }
 
 // Size the grid: persistent teams up to the resident capacity of the chip.
-int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots) {
+int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStream_t st, void** gscratch) {
     DeviceInfo& di = device_info();
     const uint32_t tpb = L.teams_per_block();
     uint64_t blocks_needed = (shots + tpb - 1) / tpb;
@@ -359,7 +394,7 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots) {
         // 16 GiB headroom, whichever is larger), at most 192 GiB
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 8ull << 30;
-        size_t budget = free_b + p->gscratch_bytes;
+        size_t budget = free_b + arena_bytes(st);
         size_t headroom = std::max<size_t>(budget * 2 / 5, 16ull << 30);
         budget = budget > headroom ? budget - headroom : (size_t)1 << 30;
         budget = std::min<size_t>(budget, 192ull << 30);
@@ -368,19 +403,20 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots) {
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, max_teams / tpb));
     }
     L.grid = (uint32_t)std::max<uint64_t>(1, std::min(blocks_needed, cap));
+    *gscratch = nullptr;
     if (L.global_amps) {
         size_t need = (size_t)L.grid * tpb * ((size_t)16 << p->peak_rank);
-        int rc = ensure_scratch(p, need);
+        int rc = ensure_scratch(st, need, gscratch);
         if (rc) return rc;
     }
     return CLIFFT_OK;
 }
 
-void fill_common_args(clifft_svm_program* p, ShotArgs& A) {
+void fill_common_args(clifft_svm_program* p, ShotArgs& A, void* gscratch) {
     std::memset(&A, 0, sizeof(A));
     A.rec_stride = (uint32_t)rec_stride_of(p);
     A.amp_elems = 1u << p->peak_rank;
-    A.gscratch = reinterpret_cast<dv2*>(p->gscratch);
+    A.gscratch = reinterpret_cast<dv2*>(gscratch);
     A.dust_counter = p->dust_dev;
 }
 
@@ -502,7 +538,6 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     }
     hipError_t e = hipMalloc(&p->dev_blob, total);
     if (e == hipSuccess) e = hipMemcpy(p->dev_blob, staging.data(), total, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipStreamCreate(&p->stream);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->dust_dev), sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(p->dust_dev, 0, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->args_dev), kArgSlots * sizeof(ShotArgs));
@@ -564,13 +599,12 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
 
 int clifft_svm_program_destroy(clifft_svm_program* p) {
     if (!p) return CLIFFT_OK;
-    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    // launches of this program may still be queued on any stream
+    (void)hipDeviceSynchronize();
     if (p->dev_blob) (void)hipFree(p->dev_blob);
-    if (p->gscratch) (void)hipFree(p->gscratch);
     if (p->out_dev) (void)hipFree(p->out_dev);
     if (p->dust_dev) (void)hipFree(p->dust_dev);
     if (p->args_dev) (void)hipFree(p->args_dev);
-    if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return CLIFFT_OK;
 }
@@ -600,10 +634,11 @@ static int run_shots(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint3
     for (uint64_t base = 0; base < shots; base += chunk_max) {
         const uint64_t n = std::min(chunk_max, shots - base);
         LaunchShape L = p->base_shape;
-        int rc = finalize_shape(p, L, n);
+        void* gs = nullptr;
+        int rc = finalize_shape(p, L, n, st, &gs);
         if (rc) return rc;
         ShotArgs A;
-        fill_common_args(p, A);
+        fill_common_args(p, A, gs);
         A.shot_begin = base;
         A.shot_count = n;
         A.seed = seed;
@@ -662,7 +697,7 @@ int clifft_svm_sample(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint
                       const clifft_svm_records* out, void* stream) {
     if (!p) return fail(CLIFFT_E_INVALID, "null program");
     if (shots == 0) return CLIFFT_OK;
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     int rc = run_shots(p, shots, seed, flags, MODE_SAMPLE, out, nullptr, nullptr, st);
     if (rc) return rc;
     if (flags & CLIFFT_REC_DEVICE) HIP_TRY(hipStreamSynchronize(st));
@@ -695,7 +730,7 @@ int clifft_svm_sample_survivors(clifft_svm_program* p, uint64_t shots, uint64_t 
         r.exp_vals = out->exp_vals ? ev.data() : nullptr;
     }
     // observables are always fetched unpacked for counting
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     int rc = run_shots(p, shots, seed, flags & ~CLIFFT_REC_PACKED, MODE_SAMPLE, &r, nullptr, nullptr, st);
     if (rc) return rc;
     const size_t bm = row_bytes(p->num_meas, false), bd = row_bytes(p->num_det, false);
@@ -742,7 +777,7 @@ int clifft_svm_record_log_probs(clifft_svm_program* p, const uint8_t* records, u
     if (p->total_slots != p->num_meas)
         return fail(CLIFFT_E_INVALID, "record_log_probs: program has hidden measurement slots");
     if (num_records == 0) return CLIFFT_OK;
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     return run_shots(p, num_records, 0, 0, MODE_FORCED, nullptr, records, log_probs, st);
 }
 
@@ -751,7 +786,7 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
     if (s->active_k > p->peak_rank) return fail(CLIFFT_E_INVALID, "state active_k exceeds peak_rank");
     if (p->has_forced_ops && !s->forced_record)
         return fail(CLIFFT_E_INVALID, "program contains *_FORCED opcodes but the state has no forced_record");
-    hipStream_t st = p->stream;
+    hipStream_t st = library_stream();
     const size_t namp = (size_t)1 << p->peak_rank;
     const size_t nrec = (size_t)p->total_slots + p->num_det + p->num_obs;
     PresetState ps{};
@@ -788,10 +823,11 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
     if (s->exp_vals && p->num_ev) HIP_TRY(hipMemcpyAsync(d_ev, s->exp_vals, p->num_ev * 8, hipMemcpyHostToDevice, st));
     if (s->forced_record) HIP_TRY(hipMemcpyAsync(d_forced, s->forced_record, p->total_slots, hipMemcpyHostToDevice, st));
     LaunchShape L = p->base_shape;
-    rc = finalize_shape(p, L, 1);
+    void* gs = nullptr;
+    rc = finalize_shape(p, L, 1, st, &gs);
     if (rc) return rc;
     ShotArgs A;
-    fill_common_args(p, A);
+    fill_common_args(p, A, gs);
     A.shot_begin = 0;
     A.shot_count = 1;
     A.mode = MODE_PRESET;
@@ -826,13 +862,497 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
     return CLIFFT_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched persistent executor
+// ---------------------------------------------------------------------------
+struct clifft_svm_batch {
+    std::vector<clifft_svm_program*> progs;
+    std::vector<uint64_t> shots;
+    std::vector<ShotArgs> args;  // host copy of the per-program launch arguments
+    uint32_t flags = 0;
+    int nw = 1;
+    clifft_svm_batch_tuning tuning{};
+    bool tuned_valid = false;
+    // device layout: [BatchArgs | counter | progs | ShotArgs[n] | seeds[n]], items separately
+    void* dev = nullptr;
+    BatchItem* items_dev = nullptr;
+    size_t items_cap = 0;
+    uint32_t n_items = 0;
+    BatchArgs* args_dev = nullptr;
+    uint32_t* counter_dev = nullptr;
+    uint64_t* seeds_dev = nullptr;
+    void* gscratch = nullptr;
+    size_t gscratch_bytes = 0;
+    uint32_t grid = 0;
+    uint32_t block = 256;
+    size_t lds_bytes = 0;
+    uint64_t seed_cache_valid = 0;
+    // captured graph
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipStream_t graph_stream = nullptr;
+    uint64_t graph_epoch = 0, epoch = 0;
+    // pinned seed staging: a ring of slots, each reused only after the copy
+    // that last read it has completed
+    static constexpr int kSeedSlots = 4;
+    uint64_t* seeds_host = nullptr;
+    hipEvent_t seed_done[kSeedSlots] = {};
+    int seed_slot = 0;
+};
+
+namespace {
+
+uint32_t tier_of(const clifft_svm_program* p, const clifft_svm_batch_tuning& t) {
+    if (p->peak_rank <= t.wave_rank_max) return TIER_WAVE;
+    if (p->peak_rank <= t.lds_rank_max) return TIER_LDS;
+    return TIER_GLOBAL;
+}
+
+size_t lds_need(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
+    const size_t waves = block / 64;
+    size_t lds = 16 * 4 * sizeof(double) + waves * rec_stride_of(p);
+    if (tier == TIER_WAVE) lds += waves * ((size_t)16 << p->peak_rank);
+    if (tier == TIER_LDS) lds += (size_t)16 << p->peak_rank;
+    return lds;
+}
+
+// Estimated workgroup time of one shot (arbitrary units): the interpreter's
+// per-instruction issue cost plus the state traffic spread over the team.
+double shot_cost(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
+    const double lanes = tier == TIER_WAVE ? 64.0 : (double)block;
+    const double per_byte = tier == TIER_GLOBAL ? 1.0 / 8.0 : 1.0 / 32.0;
+    double c = (double)p->host_bc.size() + p->bytes_per_shot / lanes * per_byte;
+    if (tier == TIER_WAVE) c /= (double)(block / 64);  // the workgroup's waves share an item
+    return c;
+}
+
+int batch_check_tuning(const clifft_svm_batch_tuning& t) {
+    if (t.block_threads != 256 && t.block_threads != 512)
+        return fail(CLIFFT_E_INVALID, "batch tuning: block_threads must be 256 or 512");
+    if (t.wave_rank_max > 8 || t.lds_rank_max > 13 || t.lds_rank_max < t.wave_rank_max)
+        return fail(CLIFFT_E_INVALID, "batch tuning: need wave_rank_max <= lds_rank_max, wave_rank_max <= 8, lds_rank_max <= 13");
+    if (t.wave_chunk == 0 || t.block_chunk == 0) return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
+    return CLIFFT_OK;
+}
+
+// Build the work queue and launch geometry for the batch's tuning, capping
+// each program at `cap` shots (the autotuner's sample); upload it.
+int batch_plan(clifft_svm_batch* b, uint64_t cap, hipStream_t st) {
+    const clifft_svm_batch_tuning& t = b->tuning;
+    int rc = batch_check_tuning(t);
+    if (rc) return rc;
+    DeviceInfo& di = device_info();
+    const uint32_t block = t.block_threads;
+    size_t lds = 16 * 4 * sizeof(double);
+    uint32_t kglobal = 0;
+    bool any_global = false;
+    struct Pending {
+        BatchItem it;
+        double cost;
+    };
+    std::vector<Pending> work;
+    for (size_t i = 0; i < b->progs.size(); ++i) {
+        clifft_svm_program* p = b->progs[i];
+        uint32_t tier = tier_of(p, t);
+        if (tier != TIER_GLOBAL && lds_need(p, tier, block) > 160 * 1024 - 64) tier = TIER_GLOBAL;
+        lds = std::max(lds, lds_need(p, tier, block));
+        if (tier == TIER_GLOBAL) {
+            any_global = true;
+            kglobal = std::max(kglobal, p->peak_rank);
+        }
+        const uint64_t n = std::min<uint64_t>(b->shots[i], cap);
+        const uint32_t chunk = tier == TIER_WAVE ? t.wave_chunk : t.block_chunk;
+        const double c = shot_cost(p, tier, block);
+        for (uint64_t lo = 0; lo < n; lo += chunk) {
+            uint64_t hi = std::min<uint64_t>(n, lo + chunk);
+            Pending w;
+            w.it.prog = (uint32_t)i;
+            w.it.tier = tier;
+            w.it.shot_lo = (uint32_t)lo;
+            w.it.shot_hi = (uint32_t)hi;
+            w.cost = c * (double)(hi - lo);
+            work.push_back(w);
+        }
+    }
+    std::stable_sort(work.begin(), work.end(), [](const Pending& a, const Pending& c) { return a.cost > c.cost; });
+    int per_cu = 0;
+    HIP_TRY(batch_occupancy(b->nw, block, lds, &per_cu));
+    if (per_cu <= 0) return fail(CLIFFT_E_UNSUPPORTED, "batch: launch shape does not fit a compute unit");
+    uint32_t grid = t.grid ? t.grid : (uint32_t)(per_cu * di.num_cus);
+    grid = std::max<uint32_t>(1, std::min<uint64_t>(grid, std::max<size_t>(work.size(), 1)));
+    const size_t gelems = any_global ? ((size_t)1 << kglobal) : 0;
+    const size_t gbytes = (size_t)grid * gelems * 16;
+    if (gbytes > b->gscratch_bytes) {
+        if (b->gscratch) {
+            HIP_TRY(hipDeviceSynchronize());
+            (void)hipFree(b->gscratch);
+        }
+        b->gscratch = nullptr;
+        b->gscratch_bytes = 0;
+        HIP_TRY(hipMalloc(&b->gscratch, gbytes));
+        b->gscratch_bytes = gbytes;
+    }
+    if (work.size() > b->items_cap) {
+        if (b->items_dev) {
+            HIP_TRY(hipDeviceSynchronize());
+            (void)hipFree(b->items_dev);
+        }
+        b->items_dev = nullptr;
+        b->items_cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b->items_dev), std::max<size_t>(work.size(), 1) * sizeof(BatchItem)));
+        b->items_cap = std::max<size_t>(work.size(), 1);
+    }
+    std::vector<BatchItem> items(work.size());
+    for (size_t i = 0; i < work.size(); ++i) items[i] = work[i].it;
+    if (!items.empty())
+        HIP_TRY(hipMemcpyAsync(b->items_dev, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+    BatchArgs B{};
+    B.items = b->items_dev;
+    B.progs = reinterpret_cast<const DevProgram* const*>(static_cast<uint8_t*>(b->dev) + 256);
+    B.args = reinterpret_cast<const ShotArgs*>(static_cast<uint8_t*>(b->dev) + 256 + align_up(b->progs.size() * 8, 256));
+    B.seeds = b->seeds_dev;
+    B.counter = b->counter_dev;
+    B.gscratch = reinterpret_cast<dv2*>(b->gscratch);
+    B.gscratch_elems = gelems;
+    B.n_items = (uint32_t)items.size();
+    HIP_TRY(hipMemcpyAsync(b->args_dev, &B, sizeof(B), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));  // host staging above is freed on return
+    b->n_items = (uint32_t)items.size();
+    b->grid = grid;
+    b->block = block;
+    b->lds_bytes = lds;
+    b->epoch++;
+    return CLIFFT_OK;
+}
+
+int batch_upload_seeds(clifft_svm_batch* b, const uint64_t* seeds, hipStream_t st) {
+    if (!seeds) return fail(CLIFFT_E_INVALID, "null seeds");
+    const size_t n = b->progs.size();
+    const int slot = b->seed_slot;
+    b->seed_slot = (slot + 1) % clifft_svm_batch::kSeedSlots;
+    HIP_TRY(hipEventSynchronize(b->seed_done[slot]));
+    uint64_t* host = b->seeds_host + (size_t)slot * n;
+    std::memcpy(host, seeds, n * sizeof(uint64_t));
+    HIP_TRY(hipMemcpyAsync(b->seeds_dev, host, n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(b->seed_done[slot], st));
+    return CLIFFT_OK;
+}
+
+clifft_svm_batch_tuning default_tuning() {
+    clifft_svm_batch_tuning t{};
+    t.block_threads = 256;
+    t.wave_rank_max = 7;
+    t.lds_rank_max = 11;
+    t.wave_chunk = 32;
+    t.block_chunk = 8;
+    t.grid = 0;
+    return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clifft_svm_batch_create(clifft_svm_program* const* programs, const uint64_t* shots, uint32_t num_programs,
+                            uint32_t flags, const clifft_svm_records* outs, clifft_svm_batch** out) {
+    if (!out) return fail(CLIFFT_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (!programs || !shots || !outs || num_programs == 0) return fail(CLIFFT_E_INVALID, "null argument or empty batch");
+    auto* b = new clifft_svm_batch();
+    b->flags = flags | CLIFFT_REC_DEVICE;
+    const bool packed = (flags & CLIFFT_REC_PACKED) != 0;
+    b->nw = 1;
+    for (uint32_t i = 0; i < num_programs; ++i) {
+        clifft_svm_program* p = programs[i];
+        if (!p) {
+            delete b;
+            return fail(CLIFFT_E_INVALID, "null program in batch");
+        }
+        if (p->has_forced_ops) {
+            delete b;
+            return fail(CLIFFT_E_INVALID, "batch: program contains *_FORCED opcodes");
+        }
+        if (shots[i] > 0xFFFFFFFFull) {
+            delete b;
+            return fail(CLIFFT_E_UNSUPPORTED, "batch: more than 2^32-1 shots for one program");
+        }
+        if (p->nw != 1) b->nw = 0;
+        b->progs.push_back(p);
+        b->shots.push_back(shots[i]);
+        ShotArgs A;
+        fill_common_args(p, A, nullptr);
+        A.shot_begin = 0;
+        A.shot_count = shots[i];
+        A.mode = MODE_SAMPLE;
+        A.flags = b->flags;
+        (void)packed;
+        A.out_meas = outs[i].measurements;
+        A.out_det = outs[i].detectors;
+        A.out_obs = outs[i].observables;
+        A.out_ev = outs[i].exp_vals;
+        A.out_disc = outs[i].discarded;
+        b->args.push_back(A);
+    }
+    const size_t n = num_programs;
+    const size_t o_progs = 256, o_args = o_progs + align_up(n * 8, 256);
+    const size_t o_seeds = o_args + align_up(n * sizeof(ShotArgs), 256);
+    const size_t o_counter = o_seeds + align_up(n * 8, 256);
+    const size_t total = o_counter + 256;
+    hipError_t e = hipMalloc(&b->dev, total);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(CLIFFT_E_HIP, std::string("batch upload: ") + hipGetErrorString(e));
+    }
+    auto* base = static_cast<uint8_t*>(b->dev);
+    b->args_dev = reinterpret_cast<BatchArgs*>(base);
+    b->seeds_dev = reinterpret_cast<uint64_t*>(base + o_seeds);
+    b->counter_dev = reinterpret_cast<uint32_t*>(base + o_counter);
+    std::vector<uint8_t> staging(o_seeds, 0);
+    for (size_t i = 0; i < n; ++i) {
+        const DevProgram* d = b->progs[i]->dp_dev;
+        std::memcpy(staging.data() + o_progs + i * 8, &d, 8);
+    }
+    std::memcpy(staging.data() + o_args, b->args.data(), n * sizeof(ShotArgs));
+    e = hipMemcpy(base, staging.data(), o_seeds, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(base + o_seeds, 0, total - o_seeds);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&b->seeds_host), clifft_svm_batch::kSeedSlots * n * sizeof(uint64_t),
+                          hipHostMallocDefault);
+    for (int i = 0; i < clifft_svm_batch::kSeedSlots && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&b->seed_done[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        clifft_svm_batch_destroy(b);
+        return fail(CLIFFT_E_HIP, std::string("batch upload: ") + hipGetErrorString(e));
+    }
+    b->tuning = default_tuning();
+    int rc = batch_plan(b, ~0ull, library_stream());
+    if (rc) {
+        std::string msg = g_last_error;
+        clifft_svm_batch_destroy(b);
+        return fail(rc, msg);
+    }
+    *out = b;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_destroy(clifft_svm_batch* b) {
+    if (!b) return CLIFFT_OK;
+    (void)hipDeviceSynchronize();
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->dev) (void)hipFree(b->dev);
+    if (b->items_dev) (void)hipFree(b->items_dev);
+    if (b->gscratch) (void)hipFree(b->gscratch);
+    if (b->seeds_host) (void)hipHostFree(b->seeds_host);
+    for (auto& ev : b->seed_done)
+        if (ev) (void)hipEventDestroy(ev);
+    delete b;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_set_tuning(clifft_svm_batch* b, const clifft_svm_batch_tuning* t) {
+    if (!b || !t) return fail(CLIFFT_E_INVALID, "null argument");
+    int rc = batch_check_tuning(*t);
+    if (rc) return rc;
+    clifft_svm_batch_tuning old = b->tuning;
+    b->tuning = *t;
+    rc = batch_plan(b, ~0ull, library_stream());
+    if (rc) {
+        std::string msg = g_last_error;
+        b->tuning = old;
+        (void)batch_plan(b, ~0ull, library_stream());
+        return fail(rc, msg);
+    }
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_get_tuning(const clifft_svm_batch* b, clifft_svm_batch_tuning* t) {
+    if (!b || !t) return fail(CLIFFT_E_INVALID, "null argument");
+    *t = b->tuning;
+    t->grid = b->tuning.grid ? b->tuning.grid : b->grid;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_run(clifft_svm_batch* b, const uint64_t* seeds, void* stream) {
+    if (!b) return fail(CLIFFT_E_INVALID, "null batch");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    int rc = batch_upload_seeds(b, seeds, st);
+    if (rc) return rc;
+    if (b->n_items == 0) return CLIFFT_OK;
+    HIP_TRY(launch_batch(b->nw, b->grid, b->block, b->lds_bytes, b->args_dev, b->counter_dev, st));
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_run_graph(clifft_svm_batch* b, const uint64_t* seeds, void* stream) {
+    if (!b) return fail(CLIFFT_E_INVALID, "null batch");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    int rc = batch_upload_seeds(b, seeds, st);
+    if (rc) return rc;
+    if (b->n_items == 0) return CLIFFT_OK;
+    if (!b->graph_exec || b->graph_stream != st || b->graph_epoch != b->epoch) {
+        if (b->graph_exec) {
+            HIP_TRY(hipStreamSynchronize(st));
+            (void)hipGraphExecDestroy(b->graph_exec);
+            b->graph_exec = nullptr;
+        }
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+        hipStream_t cap;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t le = launch_batch(b->nw, b->grid, b->block, b->lds_bytes, b->args_dev, b->counter_dev, cap);
+            e = hipStreamEndCapture(cap, &b->graph);
+            if (le != hipSuccess) e = le;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cap);
+        if (e != hipSuccess) return fail(CLIFFT_E_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
+        b->graph_stream = st;
+        b->graph_epoch = b->epoch;
+    }
+    HIP_TRY(hipGraphLaunch(b->graph_exec, st));
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stream, clifft_svm_batch_tuning* chosen) {
+    if (!b) return fail(CLIFFT_E_INVALID, "null batch");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    if (sample_shots == 0) sample_shots = ~0ull;
+    std::vector<uint64_t> seeds(b->progs.size());
+    for (size_t i = 0; i < seeds.size(); ++i) seeds[i] = 0x5eed0000ull + i;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    auto time_of = [&](const clifft_svm_batch_tuning& t, float* ms) -> int {
+        b->tuning = t;
+        int rc = batch_plan(b, sample_shots, st);
+        if (rc) return rc;
+        rc = clifft_svm_batch_run(b, seeds.data(), st);  // warm-up
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(e0, st));
+        rc = clifft_svm_batch_run(b, seeds.data(), st);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+        return CLIFFT_OK;
+    };
+    // Coordinate descent over one knob at a time, starting from the default.
+    clifft_svm_batch_tuning best = default_tuning();
+    float best_ms = 0.f;
+    int rc = time_of(best, &best_ms);
+    auto try_knob = [&](uint32_t clifft_svm_batch_tuning::*field, std::initializer_list<uint32_t> values) {
+        for (uint32_t v : values) {
+            if (rc) return;
+            clifft_svm_batch_tuning t = best;
+            t.*field = v;
+            if (t.lds_rank_max < t.wave_rank_max || batch_check_tuning(t) != CLIFFT_OK) continue;
+            float ms = 0.f;
+            int r = time_of(t, &ms);
+            if (r == CLIFFT_E_HIP) {
+                rc = r;
+                return;
+            }
+            if (r == CLIFFT_OK && ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    };
+    if (!rc) try_knob(&clifft_svm_batch_tuning::lds_rank_max, {9, 10, 12, 13});
+    if (!rc) try_knob(&clifft_svm_batch_tuning::wave_rank_max, {5, 6, 8});
+    if (!rc) try_knob(&clifft_svm_batch_tuning::block_threads, {512});
+    if (!rc) try_knob(&clifft_svm_batch_tuning::wave_chunk, {8, 16, 64, 128});
+    if (!rc) try_knob(&clifft_svm_batch_tuning::block_chunk, {2, 4, 16, 32});
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->tuning = best;
+    rc = batch_plan(b, ~0ull, st);
+    if (rc) return rc;
+    if (chosen) {
+        *chosen = best;
+        chosen->grid = b->grid;
+    }
+    return CLIFFT_OK;
+}
+
 int clifft_svm_batch_sample(clifft_svm_program* const* programs, const uint64_t* shots, const uint64_t* seeds,
                             uint32_t num_programs, uint32_t flags, const clifft_svm_records* outs, void* stream) {
     if (!programs || !shots || !seeds || !outs) return fail(CLIFFT_E_INVALID, "null argument");
-    for (uint32_t i = 0; i < num_programs; ++i) {
-        int rc = clifft_svm_sample(programs[i], shots[i], seeds[i], flags, &outs[i], stream);
-        if (rc) return rc;
+    if (num_programs == 0) return CLIFFT_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    if (flags & CLIFFT_BATCH_PER_PROGRAM) {
+        const uint32_t f = flags & ~CLIFFT_BATCH_PER_PROGRAM;
+        for (uint32_t i = 0; i < num_programs; ++i) {
+            int rc = run_shots(programs[i], shots[i], seeds[i], f, MODE_SAMPLE, &outs[i], nullptr, nullptr, st);
+            if (rc) return rc;
+        }
+        if (f & CLIFFT_REC_DEVICE) HIP_TRY(hipStreamSynchronize(st));
+        return CLIFFT_OK;
     }
+    const bool dev_out = (flags & CLIFFT_REC_DEVICE) != 0;
+    const bool packed = (flags & CLIFFT_REC_PACKED) != 0;
+    std::vector<clifft_svm_records> douts(outs, outs + num_programs);
+    void* staging = nullptr;
+    std::vector<size_t> off;
+    if (!dev_out) {
+        // one device staging buffer for every program's rows
+        size_t total = 0;
+        for (uint32_t i = 0; i < num_programs; ++i) {
+            clifft_svm_program* p = programs[i];
+            if (!p) return fail(CLIFFT_E_INVALID, "null program in batch");
+            const uint64_t n = shots[i];
+            size_t parts[5] = {outs[i].measurements ? n * row_bytes(p->num_meas, packed) : 0,
+                               outs[i].detectors ? n * row_bytes(p->num_det, packed) : 0,
+                               outs[i].observables ? n * row_bytes(p->num_obs, packed) : 0,
+                               outs[i].exp_vals ? n * p->num_ev * 8 : 0, outs[i].discarded ? n : 0};
+            for (size_t part : parts) {
+                off.push_back(total);
+                total += align_up(part, 256);
+            }
+        }
+        HIP_TRY(hipMalloc(&staging, std::max<size_t>(total, 256)));
+        auto* s = static_cast<uint8_t*>(staging);
+        for (uint32_t i = 0; i < num_programs; ++i) {
+            clifft_svm_records& r = douts[i];
+            r.measurements = outs[i].measurements ? s + off[5 * i + 0] : nullptr;
+            r.detectors = outs[i].detectors ? s + off[5 * i + 1] : nullptr;
+            r.observables = outs[i].observables ? s + off[5 * i + 2] : nullptr;
+            r.exp_vals = outs[i].exp_vals ? reinterpret_cast<double*>(s + off[5 * i + 3]) : nullptr;
+            r.discarded = outs[i].discarded ? s + off[5 * i + 4] : nullptr;
+        }
+    }
+    clifft_svm_batch* b = nullptr;
+    int rc = clifft_svm_batch_create(programs, shots, num_programs, flags, douts.data(), &b);
+    if (!rc) rc = clifft_svm_batch_run(b, seeds, st);
+    if (!rc && !dev_out) {
+        for (uint32_t i = 0; i < num_programs && !rc; ++i) {
+            clifft_svm_program* p = programs[i];
+            const uint64_t n = shots[i];
+            auto cp_out = [&](void* dst, const void* src, size_t bytes) {
+                if (rc || !dst || !bytes) return;
+                hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+                if (e != hipSuccess) rc = fail(CLIFFT_E_HIP, std::string("batch copy-out: ") + hipGetErrorString(e));
+            };
+            cp_out(outs[i].measurements, douts[i].measurements, n * row_bytes(p->num_meas, packed));
+            cp_out(outs[i].detectors, douts[i].detectors, n * row_bytes(p->num_det, packed));
+            cp_out(outs[i].observables, douts[i].observables, n * row_bytes(p->num_obs, packed));
+            cp_out(outs[i].exp_vals, douts[i].exp_vals, n * p->num_ev * 8);
+            cp_out(outs[i].discarded, douts[i].discarded, n);
+        }
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = fail(CLIFFT_E_HIP, std::string("batch: ") + hipGetErrorString(e));
+    }
+    std::string msg = g_last_error;
+    if (b) clifft_svm_batch_destroy(b);
+    if (staging) (void)hipFree(staging);
+    if (rc) return fail(rc, msg);
     return CLIFFT_OK;
 }
 

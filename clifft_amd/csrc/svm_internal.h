@@ -27,4 +27,10 @@ struct LaunchShape {
 // P and A are device memory.
 hipError_t launch_shots(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st);
 
+// Batched persistent executor (B and counter are device memory; the counter
+// is cleared by a fill kernel on `st` right before the launch).
+hipError_t batch_occupancy(int nw, uint32_t block, size_t lds_bytes, int* blocks_per_cu);
+hipError_t launch_batch(int nw, uint32_t grid, uint32_t block, size_t lds_bytes, const BatchArgs* B,
+                        uint32_t* counter, hipStream_t st);
+
 }  // namespace csvm

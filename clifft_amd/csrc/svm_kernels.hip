@@ -23,8 +23,13 @@ struct FrameOf<FRAME_LANES> {
 //   [teams_per_block x amp_elems x 16 B]  amplitudes (LDS tier only)
 //   [512 B]                               team reduction scratch (16 x 4 f64)
 //   [waves_per_block x rec_stride]        per-wavefront record copies
+//
+// The calling team runs shots s_first, s_first + s_step, ... < s_end of the
+// program; `gamp` is the team's global amplitude slice (GA only) and `seed`
+// the program's sampling seed.
 template <int FK, bool BT, bool GA>
-__device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag) {
+__device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint64_t s_first, uint64_t s_end,
+                                           uint64_t s_step, dv2* gamp, uint64_t seed) {
     typedef typename FrameOf<FK>::type F;
     typedef typename std::conditional<GA, glb_d2, lds_d2>::type AmpT;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -65,15 +70,12 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
         S.det = S.meas + P->total_meas_slots;
         S.obs = S.det + P->num_det;
     }
-    const uint64_t team_global = (uint64_t)blockIdx.x * teams_per_block + team_in_block;
-    const uint64_t nteams = (uint64_t)gridDim.x * teams_per_block;
     if (GA)
-        S.amp = (AmpT*)(A->gscratch + team_global * amp_elems);
+        S.amp = (AmpT*)(gamp);
     else
         S.amp = (AmpT*)((dv2*)lds_amp + (size_t)team_in_block * amp_elems);
 
-    const uint64_t shot_count = A->shot_count;
-    for (uint64_t s = team_global; s < shot_count; s += nteams) {
+    for (uint64_t s = s_first; s < s_end; s += s_step) {
         A = opaque(Ag);
         cptr<DevProgram> P = opaque(Pg);
         const uint64_t shot = A->shot_begin + s;
@@ -110,7 +112,7 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
             S.g_re = 1.0;
             S.g_im = 0.0;
             S.k = 0;
-            S.rng = rng_for_shot(A->seed, shot);
+            S.rng = rng_for_shot(seed, shot);
             if (S.g.lane == 0) S.amp[0] = mk(1.0, 0.0);
             S.next_noise = 0;
             S.draw_next_noise(P);
@@ -207,9 +209,67 @@ __device__ __forceinline__ void shot_body(cptr<DevProgram> Pg, cptr<ShotArgs> Ag
     }
 }
 
+// One program: teams stride over the launch's shots.
 template <int FK, bool BT, bool GA>
 __global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram* P, const ShotArgs* A) {
-    shot_body<FK, BT, GA>(cp(P), cp(A));
+    cptr<ShotArgs> Ag = cp(A);
+    const uint32_t tpb = BT ? 1u : (blockDim.x >> 6);
+    const uint32_t team_in_block = BT ? 0u : uni(threadIdx.x >> 6);
+    const uint64_t team_global = (uint64_t)blockIdx.x * tpb + team_in_block;
+    const uint64_t nteams = (uint64_t)gridDim.x * tpb;
+    dv2* gamp = GA ? Ag->gscratch + team_global * Ag->amp_elems : nullptr;
+    shot_range<FK, BT, GA>(cp(P), Ag, team_global, Ag->shot_count, nteams, gamp, Ag->seed);
+}
+
+// One work item of the batched executor, kept out of line: the caller's
+// queue loop and the three team shapes then each get their own register
+// allocation (the save/restore at the call happens once per item).
+template <int FK, bool BT, bool GA>
+__device__ __noinline__ void batch_item(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint32_t lo, uint32_t hi,
+                                        uint32_t step, dv2* gamp, uint64_t seed) {
+    shot_range<FK, BT, GA>(Pg, Ag, lo, hi, step, gamp, seed);
+}
+
+// Many programs in one persistent launch.  Each workgroup pops work items
+// (a shot range of one program, in decreasing order of estimated cost) from
+// a device queue head until the queue is empty; the item's tier picks the
+// team shape: wavefront teams with LDS amplitudes, one workgroup team with
+// LDS amplitudes, or one workgroup team with amplitudes in the workgroup's
+// global slice.  Every workgroup leaves the loop once the head passes the
+// item count, so the grid always drains.
+template <int FK>
+__global__ void __launch_bounds__(512, 4) batch_kernel(const BatchArgs* B) {
+    __shared__ uint32_t head;
+    cptr<BatchArgs> Bg = cp(B);
+    for (;;) {
+        if (threadIdx.x == 0) head = atomicAdd(Bg->counter, 1u);
+        __syncthreads();
+        const uint32_t it = uni(head);
+        __syncthreads();
+        cptr<BatchArgs> Bc = opaque(Bg);
+        if (it >= Bc->n_items) break;
+        cptr<BatchItem> I = cp(Bc->items) + it;
+        const uint32_t pid = I->prog, tier = I->tier;
+        const uint32_t lo = I->shot_lo, hi = I->shot_hi;
+        cptr<DevProgram> Pg = cp(cp(Bc->progs)[pid]);
+        cptr<ShotArgs> Ag = cp(Bc->args) + pid;
+        const uint64_t seed = cp(Bc->seeds)[pid];
+        if (tier == TIER_WAVE) {
+            const uint32_t w = uni(threadIdx.x >> 6);
+            batch_item<FK, false, false>(Pg, Ag, lo + w, hi, blockDim.x >> 6, nullptr, seed);
+        } else if (tier == TIER_LDS) {
+            batch_item<FK, true, false>(Pg, Ag, lo, hi, 1, nullptr, seed);
+        } else {
+            dv2* gamp = Bc->gscratch + (uint64_t)blockIdx.x * Bc->gscratch_elems;
+            batch_item<FK, true, true>(Pg, Ag, lo, hi, 1, gamp, seed);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void clear_u32_kernel(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -241,6 +301,41 @@ hipError_t launch_shots(const LaunchShape& L, const DevProgram* P, const ShotArg
         case 1: return launch_fk<FRAME_W1>(L, P, A, st);
         default: return launch_fk<FRAME_LANES>(L, P, A, st);
     }
+}
+
+template <int FK>
+static const void* batch_fn() {
+    return reinterpret_cast<const void*>(batch_kernel<FK>);
+}
+
+hipError_t batch_prepare(int nw) {
+    static bool done[2] = {false, false};
+    const void* fn = nw == 1 ? batch_fn<FRAME_W1>() : batch_fn<FRAME_LANES>();
+    if (done[nw == 1]) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess) done[nw == 1] = true;
+    return e;
+}
+
+hipError_t batch_occupancy(int nw, uint32_t block, size_t lds_bytes, int* blocks_per_cu) {
+    hipError_t e = batch_prepare(nw);
+    if (e != hipSuccess) return e;
+    const void* fn = nw == 1 ? batch_fn<FRAME_W1>() : batch_fn<FRAME_LANES>();
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, (int)block, lds_bytes);
+}
+
+hipError_t launch_batch(int nw, uint32_t grid, uint32_t block, size_t lds_bytes, const BatchArgs* B,
+                        uint32_t* counter, hipStream_t st) {
+    hipError_t e = batch_prepare(nw);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(clear_u32_kernel, dim3(1), dim3(64), 0, st, counter, 1u);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (nw == 1)
+        hipLaunchKernelGGL(batch_kernel<FRAME_W1>, dim3(grid), dim3(block), lds_bytes, st, B);
+    else
+        hipLaunchKernelGGL(batch_kernel<FRAME_LANES>, dim3(grid), dim3(block), lds_bytes, st, B);
+    return hipGetLastError();
 }
 
 }  // namespace csvm

@@ -300,8 +300,11 @@ int clifft_svm_program_info_get(const clifft_svm_program* prog, clifft_svm_progr
  * xoshiro256++ stream: the SplitMix64 sequence seeded with `seed`, outputs
  * 4s .. 4s+3 (so shot 0 draws exactly what the reference's single stream
  * draws for its first shot).  Observables are XORed with the program's
- * expected_observables.  `stream` is a hipStream_t (NULL = the program's own
- * stream); the call returns after the records are written. */
+ * expected_observables.  `stream` is a hipStream_t (NULL = the library's own
+ * stream); the call returns after the records are written.  Programs whose
+ * amplitudes live in global memory share one scratch arena per stream, so
+ * launches on one stream never overlap in it; one program must not be run
+ * concurrently on two streams. */
 int clifft_svm_sample(clifft_svm_program* prog, uint64_t shots, uint64_t seed, uint32_t flags,
                       const clifft_svm_records* out, void* stream);
 
@@ -327,11 +330,56 @@ int clifft_svm_record_log_probs(clifft_svm_program* prog, const uint8_t* records
 int clifft_svm_execute(clifft_svm_program* prog, clifft_svm_state* state);
 
 /* --- batching ---------------------------------------------------------------*/
+/* No counterpart in the reference (its VM runs one program per call,
+ * src/clifft/svm/svm.h:302-306).  Many programs of uneven peak rank and
+ * bytecode length run in ONE persistent launch: work items (a shot range of
+ * one program) are sorted by descending estimated cost and drained by the
+ * workgroups from a device queue.  Each item runs with the team shape of
+ * its tier: one shot per wavefront (peak_rank <= wave_rank_max), one shot
+ * per workgroup with LDS-resident amplitudes (<= lds_rank_max), or one shot
+ * per workgroup with amplitudes in a per-workgroup global slice.
+ *
+ * Shot s of program i draws from the per-shot stream of (seeds[i], s), as in
+ * clifft_svm_sample, so the records equal that call's; the team shape (and
+ * so the order of floating-point reductions) may differ from the plain
+ * launch, which moves expectation values by rounding only.
+ * Outputs of a batch are device memory (CLIFFT_REC_DEVICE is implied). */
 
-/* Run many programs in one persistent launch.  programs[i] runs shots[i]
- * shots with seed seeds[i] into outs[i] (same layout as clifft_svm_sample).
- * Work items (program, shot range) are drained from a device queue ordered
- * by descending cost. */
+typedef struct clifft_svm_batch clifft_svm_batch;
+
+typedef struct {
+    uint32_t block_threads; /* 256 or 512 */
+    uint32_t wave_rank_max; /* <= 8 */
+    uint32_t lds_rank_max;  /* <= 13, >= wave_rank_max */
+    uint32_t wave_chunk;    /* shots per work item, wavefront tier */
+    uint32_t block_chunk;   /* shots per work item, workgroup tiers */
+    uint32_t grid;          /* workgroups (0 = resident capacity of the device) */
+} clifft_svm_batch_tuning;
+
+/* outs[i] are device pointers; CLIFFT_REC_PACKED is honoured. */
+int clifft_svm_batch_create(clifft_svm_program* const* programs, const uint64_t* shots,
+                            uint32_t num_programs, uint32_t flags, const clifft_svm_records* outs,
+                            clifft_svm_batch** out);
+int clifft_svm_batch_destroy(clifft_svm_batch* batch);
+int clifft_svm_batch_set_tuning(clifft_svm_batch* batch, const clifft_svm_batch_tuning* tuning);
+int clifft_svm_batch_get_tuning(const clifft_svm_batch* batch, clifft_svm_batch_tuning* tuning);
+/* Autotuner: times candidate tunings on the batch itself (each program
+ * capped at `sample_shots` shots, outputs overwritten) with HIP events on
+ * `stream`, keeps the fastest and returns it in `chosen` (may be NULL). */
+int clifft_svm_batch_tune(clifft_svm_batch* batch, uint64_t sample_shots, void* stream,
+                          clifft_svm_batch_tuning* chosen);
+/* One persistent launch with per-program seeds (host array).  Asynchronous
+ * on `stream` (NULL = the library stream). */
+int clifft_svm_batch_run(clifft_svm_batch* batch, const uint64_t* seeds, void* stream);
+/* Same work through a captured HIP graph (queue reset + persistent launch),
+ * captured on first use per stream and tuning and replayed afterwards. */
+int clifft_svm_batch_run_graph(clifft_svm_batch* batch, const uint64_t* seeds, void* stream);
+
+/* One-call form.  Without CLIFFT_BATCH_PER_PROGRAM it builds a batch (with
+ * the default tuning), runs it once and destroys it; outputs may be host
+ * memory (staged) or device memory.  With CLIFFT_BATCH_PER_PROGRAM it issues
+ * one plain clifft_svm_sample launch per program (the unbatched baseline). */
+#define CLIFFT_BATCH_PER_PROGRAM 0x100u
 int clifft_svm_batch_sample(clifft_svm_program* const* programs, const uint64_t* shots,
                             const uint64_t* seeds, uint32_t num_programs, uint32_t flags,
                             const clifft_svm_records* outs, void* stream);

@@ -1,0 +1,200 @@
+"""Batched persistent executor vs per-program launches and the oracle.
+
+Every program of a batch must produce the records `svm.sample(program,
+shots, seed)` produces (the per-shot stream of (seed, shot) is the same; the
+team shape, and so the reduction order, can differ between the two paths,
+which moves expectation values by rounding only).
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from randprog import random_program
+
+pytestmark = pytest.mark.gpu
+
+ATOL = 1e-12
+
+
+def announce(*a):
+    print("CASE", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import clifft_amd.batch as batch
+    import clifft_amd.svm as svm
+    from clifft_amd import _lib
+    assert _lib.backend() == "hip:gfx950", _lib.backend()
+    return svm, batch
+
+
+# peaks spanning every tier: wavefront teams, LDS workgroup teams, global
+PEAKS = [2, 4, 6, 7, 8, 9, 11, 12, 13, 14, 15]
+
+
+def mixed_programs(seed, count, wide=False):
+    rng = np.random.default_rng(seed)
+    progs, shots, seeds = [], [], []
+    for i in range(count):
+        peak = PEAKS[i % len(PEAKS)]
+        nq = int(rng.integers(peak + 2, peak + 14)) if not wide or i % 3 else 70 + i
+        progs.append(random_program(seed * 1000 + i, num_qubits=nq, peak=peak, length=int(rng.integers(60, 220))))
+        shots.append(int(rng.integers(1, 90)) if peak >= 12 else int(rng.integers(1, 400)))
+        seeds.append(int(rng.integers(0, 2**63)))
+    return progs, shots, seeds
+
+
+def assert_same(a, b, what):
+    assert np.array_equal(a.measurements, b.measurements), what
+    assert np.array_equal(a.detectors, b.detectors), what
+    assert np.array_equal(a.observables, b.observables), what
+    np.testing.assert_allclose(a.exp_vals, b.exp_vals, atol=ATOL, rtol=0, err_msg=what)
+
+
+@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("packed", [False, True])
+def test_batch_matches_plain(mods, wide, packed):
+    svm, batch = mods
+    announce("batch-vs-plain", wide, packed)
+    progs, shots, seeds = mixed_programs(11 + wide, 22, wide=wide)
+    got = batch.batch_sample(progs, shots, seeds, packed=packed)
+    plain = batch.batch_sample(progs, shots, seeds, packed=packed, per_program=True)
+    for i, p in enumerate(progs):
+        ref = svm.sample(p, shots[i], seed=seeds[i], packed=packed)
+        assert_same(got[i], ref, f"program {i} (peak {p.peak_rank}) persistent")
+        assert_same(plain[i], ref, f"program {i} (peak {p.peak_rank}) per-program")
+
+
+def test_batch_matches_oracle(mods, orc):
+    svm, batch = mods
+    announce("batch-vs-oracle")
+    progs, shots, seeds = mixed_programs(5, 11)
+    got = batch.batch_sample(progs, shots, seeds)
+    for i, p in enumerate(progs):
+        m, d, o, e, _ = orc.sample(p, shots[i], seed=seeds[i], per_shot=True)
+        assert np.array_equal(got[i].measurements, m), i
+        assert np.array_equal(got[i].detectors, d), i
+        assert np.array_equal(got[i].observables, o), i
+        np.testing.assert_allclose(got[i].exp_vals, e, atol=ATOL, rtol=0)
+
+
+def _device_batch(batch, progs, shots, packed=False):
+    import torch
+    dev = torch.device("cuda:0")
+    cols = (lambda n: (n + 7) // 8) if packed else (lambda n: n)
+    bufs, outs = [], []
+    for p, s in zip(progs, shots):
+        m = torch.zeros((s, cols(p.num_measurements)), dtype=torch.uint8, device=dev)
+        d = torch.zeros((s, cols(p.num_detectors)), dtype=torch.uint8, device=dev)
+        o = torch.zeros((s, cols(p.num_observables)), dtype=torch.uint8, device=dev)
+        e = torch.zeros((s, p.num_exp_vals), dtype=torch.float64, device=dev)
+        bufs.append((m, d, o, e))
+        outs.append({"measurements": m.data_ptr() if m.numel() else None,
+                     "detectors": d.data_ptr() if d.numel() else None,
+                     "observables": o.data_ptr() if o.numel() else None,
+                     "exp_vals": e.data_ptr() if e.numel() else None})
+    return batch.Batch(progs, shots, outs, packed=packed), bufs
+
+
+def _snapshot(bufs):
+    return [tuple(t.cpu().numpy().copy() for t in b) for b in bufs]
+
+
+TUNINGS = [dict(block_threads=512), dict(lds_rank_max=9), dict(lds_rank_max=13), dict(wave_rank_max=5),
+           dict(wave_rank_max=8, lds_rank_max=8), dict(wave_chunk=4, block_chunk=1), dict(grid=7),
+           dict(block_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32)]
+
+
+def test_batch_tunings_agree(mods):
+    import torch
+    svm, batch = mods
+    announce("batch-tunings")
+    progs, shots, seeds = mixed_programs(21, 22)
+    b, bufs = _device_batch(batch, progs, shots)
+    b.run(seeds)
+    torch.cuda.synchronize()
+    ref = _snapshot(bufs)
+    for t in TUNINGS:
+        announce("tuning", t)
+        for buf in bufs:
+            for x in buf:
+                x.zero_()
+        b.set_tuning(**t)
+        b.run(seeds)
+        torch.cuda.synchronize()
+        got = _snapshot(bufs)
+        for i in range(len(progs)):
+            for j in range(3):
+                assert np.array_equal(got[i][j], ref[i][j]), (t, i, j)
+            np.testing.assert_allclose(got[i][3], ref[i][3], atol=ATOL, rtol=0)
+    b.close()
+
+
+def test_batch_autotune(mods):
+    import torch
+    svm, batch = mods
+    announce("batch-autotune")
+    progs, shots, seeds = mixed_programs(31, 33)
+    b, bufs = _device_batch(batch, progs, shots, packed=True)
+    chosen = b.tune(sample_shots=16)
+    assert chosen["block_threads"] in (256, 512) and chosen["grid"] > 0
+    assert chosen["wave_rank_max"] <= chosen["lds_rank_max"] <= 13
+    b.run(seeds)
+    torch.cuda.synchronize()
+    got = _snapshot(bufs)
+    for i, p in enumerate(progs):
+        ref = svm.sample(p, shots[i], seed=seeds[i], packed=True)
+        assert np.array_equal(got[i][0], ref.measurements), i
+        assert np.array_equal(got[i][1], ref.detectors), i
+        assert np.array_equal(got[i][2], ref.observables), i
+    b.close()
+
+
+def test_batch_graph_replays(mods):
+    """Captured graph replays, with eager work and synchronizations between
+    them, each match an eager run with the same seeds."""
+    import torch
+    svm, batch = mods
+    announce("batch-graph")
+    progs, shots, seeds = mixed_programs(41, 15)
+    b, bufs = _device_batch(batch, progs, shots)
+    rng = np.random.default_rng(3)
+    for rep in range(4):
+        sd = [int(x) for x in rng.integers(0, 2**63, size=len(progs))]
+        b.run(sd)
+        torch.cuda.synchronize()
+        eager = _snapshot(bufs)
+        for buf in bufs:
+            for x in buf:
+                x.fill_(7)
+        torch.cuda.synchronize()
+        b.run_graph(sd)
+        torch.cuda.synchronize()
+        graph = _snapshot(bufs)
+        for i in range(len(progs)):
+            for j in range(3):
+                assert np.array_equal(graph[i][j], eager[i][j]), (rep, i, j)
+            np.testing.assert_allclose(graph[i][3], eager[i][3], atol=0, rtol=0)
+        _ = torch.ones(1000, device="cuda:0").sum().item()  # unrelated eager work between replays
+    b.close()
+
+
+def test_batch_rejects_bad_tuning(mods):
+    svm, batch = mods
+    from clifft_amd._lib import SvmError
+    progs, shots, seeds = mixed_programs(51, 3)
+    b, bufs = _device_batch(batch, progs, shots)
+    with pytest.raises(SvmError):
+        b.set_tuning(block_threads=300)
+    with pytest.raises(SvmError):
+        b.set_tuning(wave_rank_max=9)
+    with pytest.raises(SvmError):
+        b.set_tuning(lds_rank_max=14)
+    b.run(seeds)  # still usable with the previous tuning
+    b.close()
