@@ -39,6 +39,8 @@ WORKLOADS = {
     "surface_d3": ("noisy d=3 rotated surface-code memory, 3 rounds, p=1e-3, 1 T/round (17 qubits)", 1_000_000),
     "clifford_t20": ("20 qubits x 40 layers random Clifford+T, 6 T, final M (peak k=6)", 100_000),
     "large_k34": ("34 qubits x 40 layers, 26 T on distinct qubits, MR on 4 qubits / 10 layers", 256),
+    "rz24_grad": ("24 qubits, 30 Clifford layers, 16 R_Z(theta_i) slots (peak k=16), 32 random Pauli probes; "
+                  "<O> and d<O>/dtheta for 512 theta vectors by the adjoint kernel", 512),
     "batch4096": ("4096 random Clifford+T programs, n ~ U[16,40], T-count ~ U[4,16], 40 layers, final M; "
                   "one persistent launch", 1000),
 }
@@ -108,6 +110,97 @@ def cpu_baseline(prog, run_seconds=2.5, max_shots=20_000_000):
                       f"host {host}",
             "mode_b": {"value": n_b / t_b, "unit": "shots/s", "cores": threads,
                        "sample": f"{n_b} shots, OpenMP over shots, per-shot seeds, median of 3 = {t_b:.2f} s"}}
+
+
+def run_grad(args):
+    """Config: 512 theta vectors of the 24-qubit / 16-parameter / 32-probe
+    circuit; <O> and its gradient by one forward + one backward sweep per
+    vector, against GPU parameter-shift (2P+1 forward evaluations per vector
+    with the same kernel in forward-only mode)."""
+    import torch
+    from clifft_amd import _lib, grad
+    from clifft_amd import workloads as W
+    from clifft_amd.compile import compile as ccompile
+
+    desc, B = WORKLOADS["rz24_grad"]
+    if args.shots:
+        B = args.shots
+    build, coeffs, _ = W.rz_exp_val()
+    rng = np.random.default_rng(42)
+    thetas = rng.uniform(0, 2 * np.pi, (B, 16))
+    prog = ccompile(build(thetas[0]), keep_rotations=True)
+    adj = grad.Adjoint(prog)
+    P, nev = adj.num_params, adj.num_exp_vals
+    dev = torch.device("cuda:0")
+    th = torch.tensor(thetas, dtype=torch.float64, device=dev)
+    c = torch.tensor(coeffs, dtype=torch.float64, device=dev)
+    ev = torch.empty((B, nev), dtype=torch.float64, device=dev)
+    g = torch.empty((B, P), dtype=torch.float64, device=dev)
+    # parameter-shift inputs: for each vector, theta and theta +- pi/2 e_i
+    shift = np.repeat(thetas[:, None, :], 2 * P + 1, axis=1)
+    for i in range(P):
+        shift[:, 1 + 2 * i, i] += np.pi / 2
+        shift[:, 2 + 2 * i, i] -= np.pi / 2
+    th_ps = torch.tensor(shift.reshape(-1, P), dtype=torch.float64, device=dev)
+    ev_ps = torch.empty((th_ps.shape[0], nev), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def adjoint_step():
+        adj.run_device(th.data_ptr(), B, c.data_ptr(), ev.data_ptr(), g.data_ptr(), stream=sh)
+
+    def shift_step():
+        adj.run_device(th_ps.data_ptr(), th_ps.shape[0], None, ev_ps.data_ptr(), None, stream=sh)
+
+    def timed(fn, steps):
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                fn()
+            torch.cuda.synchronize()
+            s0 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+            s1 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+            t0 = time.perf_counter()
+            for i in range(steps):
+                s0[i].record(stream)
+                fn()
+                s1[i].record(stream)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        return wall, float(np.mean([a.elapsed_time(b) for a, b in zip(s0, s1)]))
+
+    wall, kern_ms = timed(adjoint_step, args.steps)
+    wall_ps, kern_ps = timed(shift_step, max(1, args.steps // 4))
+    steps_ps = max(1, args.steps // 4)
+    F = (ev @ c).cpu().numpy()
+    G = g.cpu().numpy()
+    Fps = (ev_ps @ c).cpu().numpy().reshape(B, 2 * P + 1)
+    Gps = 0.5 * (Fps[:, 1::2] - Fps[:, 2::2])
+    grads_per_s = B * args.steps / wall
+    ps_per_s = B * steps_ps / wall_ps
+    # algorithmic bytes per theta vector: forward state traffic, lambda
+    # construction (read phi + read/write lambda per probe), backward sweeps
+    # over phi and lambda, and one inner product per parameter
+    info = adj.dp.info()
+    fwd = info["bytes_per_shot"]
+    kfin = 16 * (1 << prog.peak_rank)
+    alg_vec = fwd + 48 * (kfin // 16) * nev + 2 * (fwd - 16 * (kfin // 16) * nev) + 32 * (kfin // 16) * P
+    achieved = alg_vec * B / (kern_ms * 1e-3) / 1e9
+    line = {
+        "metric": "grads/s", "value": grads_per_s, "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "rz24_grad", "description": desc, "theta_vectors": B, "params": P, "probes": nev,
+                   "peak_rank": prog.peak_rank, "instructions": int(len(prog.bytecode)),
+                   "param_shift_grads_per_s": ps_per_s, "speedup_vs_param_shift": grads_per_s / ps_per_s,
+                   "max_abs_diff_vs_param_shift": float(np.abs(G - Gps).max()),
+                   "max_abs_diff_value": float(np.abs(F - Fps[:, 0]).max()),
+                   "param_shift_kernel_ms": kern_ps, "parallelism": "theta vectors"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kern_ms,
+                     "algorithmic_bytes_per_launch": alg_vec * B},
+        "backend": _lib.backend(),
+    }
+    print(json.dumps(line), flush=True)
 
 
 def run_batch(args):
@@ -237,6 +330,8 @@ def main():
         return
     if args.workload == "batch4096":
         return run_batch(args)
+    if args.workload == "rz24_grad":
+        return run_grad(args)
     desc, shots = WORKLOADS[args.workload]
     if args.shots:
         shots = args.shots

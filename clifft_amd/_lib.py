@@ -56,7 +56,8 @@ class State(ctypes.Structure):
 class BatchTuning(ctypes.Structure):
     _fields_ = [("block_threads", ctypes.c_uint32), ("wave_rank_max", ctypes.c_uint32),
                 ("lds_rank_max", ctypes.c_uint32), ("wave_chunk", ctypes.c_uint32),
-                ("block_chunk", ctypes.c_uint32), ("grid", ctypes.c_uint32)]
+                ("block_chunk", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("lane_rank_max", ctypes.c_int32), ("lane_chunk", ctypes.c_uint32)]
 
 
 # name -> (restype, argtypes); exactly the functions include/clifft_svm.h declares
@@ -85,6 +86,11 @@ SIGNATURES = {
                                              ctypes.POINTER(BatchTuning)]),
     "clifft_svm_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "clifft_svm_batch_run_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "clifft_svm_adjoint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "clifft_svm_adjoint_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "clifft_svm_adjoint_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
     "clifft_svm_batch_sample": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                                ctypes.POINTER(Records), ctypes.c_void_p]),

@@ -116,9 +116,17 @@ struct ShotArgs {
     unsigned long long* dust_counter; // optional telemetry
 };
 
+// Lane tier (one shot per lane): words of bit-packed records per lane, and
+// the LDS bytes of one wavefront: 2^peak amplitude rows of 64 x 16 B, then
+// the measurement, detector and observable words, 64 x 4 B each.
+__host__ __device__ inline uint32_t lane_words(uint32_t bits) { return (bits + 31u) >> 5; }
+__host__ __device__ inline uint32_t lane_wave_bytes(uint32_t peak_rank, uint32_t slots, uint32_t ndet, uint32_t nobs) {
+    return (64u * 16u << peak_rank) + 256u * (lane_words(slots) + lane_words(ndet) + lane_words(nobs));
+}
+
 // Batched executor: one work-queue entry is a contiguous range of shots of
 // one program, run by one workgroup with the team shape named by `tier`.
-enum : uint32_t { TIER_WAVE = 0, TIER_LDS = 1, TIER_GLOBAL = 2 };
+enum : uint32_t { TIER_WAVE = 0, TIER_LDS = 1, TIER_GLOBAL = 2, TIER_LANE = 3 };
 
 struct BatchItem {
     uint32_t prog;  // index into BatchArgs::progs / args / seeds

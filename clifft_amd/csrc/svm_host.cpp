@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -312,11 +313,32 @@ size_t rec_stride_of(const clifft_svm_program* p) {
     return align_up(std::max<size_t>(p->total_slots + p->num_det + p->num_obs, 4), 16);
 }
 
+// Highest peak rank run one shot per lane (CLIFFT_LANE_MAX_RANK overrides;
+// a negative value turns the lane tier off).
+int lane_max_rank() {
+    static int v = [] {
+        const char* e = std::getenv("CLIFFT_LANE_MAX_RANK");
+        return e ? std::atoi(e) : 3;
+    }();
+    return v;
+}
+
 // Choose the team shape and storage tier for a program's peak rank.
-LaunchShape plan_shape(const clifft_svm_program* p) {
+LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true) {
     LaunchShape L;
     L.nw = (int)p->nw;
     const uint32_t k = p->peak_rank;
+    if (allow_lane && p->words == 1 && (int)k <= lane_max_rank()) {
+        // up to 4 wavefronts per workgroup, as many as the LDS holds
+        const size_t wb = lane_wave_bytes(k, p->total_slots, p->num_det, p->num_obs);
+        const size_t waves = std::min<size_t>(4, (160 * 1024) / wb);
+        if (waves > 0) {
+            L.lane = true;
+            L.block = (uint32_t)(64 * waves);
+            L.lds_bytes = waves * wb;
+            return L;
+        }
+    }
     const size_t rec_stride = rec_stride_of(p);
     const size_t amp_bytes = (size_t)16 << k;
     const size_t red = 16 * 4 * sizeof(double);
@@ -345,8 +367,6 @@ LaunchShape plan_shape(const clifft_svm_program* p) {
     return L;
 }
 
-// Grow the stream's arena to `bytes` (the old buffer may still be read by
-// queued launches, so wait for the stream before freeing it).
 int ensure_scratch(hipStream_t st, size_t bytes, void** ptr) {
     std::lock_guard<std::mutex> lock(g_arena_mu);
     Arena& a = arena_of(st);
@@ -384,10 +404,10 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStr
     DeviceInfo& di = device_info();
     const uint32_t tpb = L.teams_per_block();
     uint64_t blocks_needed = (shots + tpb - 1) / tpb;
-    uint32_t waves = L.block / 64;
-    uint32_t by_lds = L.lds_bytes > 0 ? (uint32_t)std::max<size_t>(1, (160 * 1024) / L.lds_bytes) : 32;
-    uint32_t by_waves = std::max<uint32_t>(1, 32 / waves);
-    uint32_t per_cu = std::min(by_lds, by_waves);
+    int per_cu_i = 0;
+    hipError_t oe = shot_occupancy(L, &per_cu_i);
+    if (oe != hipSuccess) return fail(CLIFFT_E_HIP, std::string("occupancy query: ") + hipGetErrorString(oe));
+    uint32_t per_cu = (uint32_t)std::max(per_cu_i, 1);
     uint64_t cap = (uint64_t)di.num_cus * per_cu;
     if (L.global_amps) {
         // bound the global scratch by what the device has free (keep 40% or
@@ -822,7 +842,7 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
     HIP_TRY(hipMemcpyAsync(d_rec, recs.data(), nrec, hipMemcpyHostToDevice, st));
     if (s->exp_vals && p->num_ev) HIP_TRY(hipMemcpyAsync(d_ev, s->exp_vals, p->num_ev * 8, hipMemcpyHostToDevice, st));
     if (s->forced_record) HIP_TRY(hipMemcpyAsync(d_forced, s->forced_record, p->total_slots, hipMemcpyHostToDevice, st));
-    LaunchShape L = p->base_shape;
+    LaunchShape L = p->base_shape.lane ? plan_shape(p, false) : p->base_shape;
     void* gs = nullptr;
     rc = finalize_shape(p, L, 1, st, &gs);
     if (rc) return rc;
@@ -905,6 +925,7 @@ struct clifft_svm_batch {
 namespace {
 
 uint32_t tier_of(const clifft_svm_program* p, const clifft_svm_batch_tuning& t) {
+    if (p->words == 1 && (int32_t)p->peak_rank <= t.lane_rank_max) return TIER_LANE;
     if (p->peak_rank <= t.wave_rank_max) return TIER_WAVE;
     if (p->peak_rank <= t.lds_rank_max) return TIER_LDS;
     return TIER_GLOBAL;
@@ -912,6 +933,7 @@ uint32_t tier_of(const clifft_svm_program* p, const clifft_svm_batch_tuning& t) 
 
 size_t lds_need(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
     const size_t waves = block / 64;
+    if (tier == TIER_LANE) return waves * lane_wave_bytes(p->peak_rank, p->total_slots, p->num_det, p->num_obs);
     size_t lds = 16 * 4 * sizeof(double) + waves * rec_stride_of(p);
     if (tier == TIER_WAVE) lds += waves * ((size_t)16 << p->peak_rank);
     if (tier == TIER_LDS) lds += (size_t)16 << p->peak_rank;
@@ -921,6 +943,8 @@ size_t lds_need(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
 // Estimated workgroup time of one shot (arbitrary units): the interpreter's
 // per-instruction issue cost plus the state traffic spread over the team.
 double shot_cost(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
+    if (tier == TIER_LANE)  // 64 shots per wavefront, the workgroup's waves share an item
+        return ((double)p->host_bc.size() + p->bytes_per_shot / 64.0) / (double)block;
     const double lanes = tier == TIER_WAVE ? 64.0 : (double)block;
     const double per_byte = tier == TIER_GLOBAL ? 1.0 / 8.0 : 1.0 / 32.0;
     double c = (double)p->host_bc.size() + p->bytes_per_shot / lanes * per_byte;
@@ -933,7 +957,9 @@ int batch_check_tuning(const clifft_svm_batch_tuning& t) {
         return fail(CLIFFT_E_INVALID, "batch tuning: block_threads must be 256 or 512");
     if (t.wave_rank_max > 8 || t.lds_rank_max > 13 || t.lds_rank_max < t.wave_rank_max)
         return fail(CLIFFT_E_INVALID, "batch tuning: need wave_rank_max <= lds_rank_max, wave_rank_max <= 8, lds_rank_max <= 13");
-    if (t.wave_chunk == 0 || t.block_chunk == 0) return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
+    if (t.wave_chunk == 0 || t.block_chunk == 0 || t.lane_chunk == 0)
+        return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
+    if (t.lane_rank_max > 5) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 5");
     return CLIFFT_OK;
 }
 
@@ -956,14 +982,14 @@ int batch_plan(clifft_svm_batch* b, uint64_t cap, hipStream_t st) {
     for (size_t i = 0; i < b->progs.size(); ++i) {
         clifft_svm_program* p = b->progs[i];
         uint32_t tier = tier_of(p, t);
-        if (tier != TIER_GLOBAL && lds_need(p, tier, block) > 160 * 1024 - 64) tier = TIER_GLOBAL;
+        if (tier != TIER_GLOBAL && lds_need(p, tier, block) > kBatchMaxDynLds) tier = TIER_GLOBAL;
         lds = std::max(lds, lds_need(p, tier, block));
         if (tier == TIER_GLOBAL) {
             any_global = true;
             kglobal = std::max(kglobal, p->peak_rank);
         }
         const uint64_t n = std::min<uint64_t>(b->shots[i], cap);
-        const uint32_t chunk = tier == TIER_WAVE ? t.wave_chunk : t.block_chunk;
+        const uint32_t chunk = tier == TIER_LANE ? t.lane_chunk : (tier == TIER_WAVE ? t.wave_chunk : t.block_chunk);
         const double c = shot_cost(p, tier, block);
         for (uint64_t lo = 0; lo < n; lo += chunk) {
             uint64_t hi = std::min<uint64_t>(n, lo + chunk);
@@ -1048,6 +1074,8 @@ clifft_svm_batch_tuning default_tuning() {
     t.wave_chunk = 32;
     t.block_chunk = 8;
     t.grid = 0;
+    t.lane_rank_max = lane_max_rank() < 0 ? -1 : std::min(lane_max_rank(), 5);
+    t.lane_chunk = 1024;
     return t;
 }
 
@@ -1262,6 +1290,25 @@ int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stre
             }
         }
     };
+    auto try_lane = [&](std::initializer_list<int32_t> values) {
+        for (int32_t v : values) {
+            if (rc) return;
+            clifft_svm_batch_tuning t = best;
+            t.lane_rank_max = v;
+            float ms = 0.f;
+            int r = time_of(t, &ms);
+            if (r == CLIFFT_E_HIP) {
+                rc = r;
+                return;
+            }
+            if (r == CLIFFT_OK && ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    };
+    if (!rc) try_lane({-1, 2, 4});
+    if (!rc) try_knob(&clifft_svm_batch_tuning::lane_chunk, {256, 512, 2048});
     if (!rc) try_knob(&clifft_svm_batch_tuning::lds_rank_max, {9, 10, 12, 13});
     if (!rc) try_knob(&clifft_svm_batch_tuning::wave_rank_max, {5, 6, 8});
     if (!rc) try_knob(&clifft_svm_batch_tuning::block_threads, {512});
@@ -1353,6 +1400,196 @@ int clifft_svm_batch_sample(clifft_svm_program* const* programs, const uint64_t*
     if (b) clifft_svm_batch_destroy(b);
     if (staging) (void)hipFree(staging);
     if (rc) return fail(rc, msg);
+    return CLIFFT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint gradients
+// ---------------------------------------------------------------------------
+struct clifft_svm_adjoint {
+    clifft_svm_program* prog = nullptr;
+    uint32_t num_params = 0;
+    void* dev = nullptr;  // [slot_of (n_instr x i32) | psign (np x f64) | AdjArgs ring]
+    int32_t* slot_of = nullptr;
+    double* psign = nullptr;
+    AdjArgs* args_dev = nullptr;
+    uint32_t args_slot = 0;
+    bool global_amps = false;
+    uint32_t block = 256;
+    size_t lds_bytes = 0;
+    int per_cu = 1;
+    void* gscratch = nullptr;
+    size_t gscratch_bytes = 0;
+    void* staging = nullptr;
+    size_t staging_bytes = 0;
+};
+
+namespace {
+
+constexpr uint32_t kAdjArgSlots = 64;
+
+bool adjoint_op_ok(uint8_t op) {
+    switch (op) {
+        case CLIFFT_OP_FRAME_CNOT: case CLIFFT_OP_FRAME_CZ: case CLIFFT_OP_FRAME_H: case CLIFFT_OP_FRAME_S:
+        case CLIFFT_OP_FRAME_S_DAG: case CLIFFT_OP_FRAME_SWAP: case CLIFFT_OP_ARRAY_CNOT: case CLIFFT_OP_ARRAY_CZ:
+        case CLIFFT_OP_ARRAY_SWAP: case CLIFFT_OP_ARRAY_MULTI_CNOT: case CLIFFT_OP_ARRAY_MULTI_CZ:
+        case CLIFFT_OP_ARRAY_H: case CLIFFT_OP_ARRAY_S: case CLIFFT_OP_ARRAY_S_DAG: case CLIFFT_OP_ARRAY_T:
+        case CLIFFT_OP_ARRAY_T_DAG: case CLIFFT_OP_ARRAY_ROT: case CLIFFT_OP_ARRAY_U2: case CLIFFT_OP_ARRAY_U4:
+        case CLIFFT_OP_EXPAND: case CLIFFT_OP_EXPAND_T: case CLIFFT_OP_EXPAND_T_DAG: case CLIFFT_OP_EXPAND_ROT:
+        case CLIFFT_OP_EXP_VAL:
+            return true;
+        default:
+            return false;
+    }
+}
+
+int adjoint_ensure(void** ptr, size_t* cur, size_t bytes) {
+    if (bytes <= *cur) return CLIFFT_OK;
+    if (*ptr) {
+        HIP_TRY(hipDeviceSynchronize());
+        (void)hipFree(*ptr);
+    }
+    *ptr = nullptr;
+    *cur = 0;
+    HIP_TRY(hipMalloc(ptr, bytes));
+    *cur = bytes;
+    return CLIFFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clifft_svm_adjoint_create(clifft_svm_program* p, const uint32_t* param_instr, const int8_t* param_sign,
+                              uint32_t num_params, clifft_svm_adjoint** out) {
+    if (!out) return fail(CLIFFT_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (!p || (num_params && (!param_instr || !param_sign))) return fail(CLIFFT_E_INVALID, "null argument");
+    const size_t n = p->host_bc.size();
+    for (size_t i = 0; i < n; ++i)
+        if (!adjoint_op_ok(p->host_bc[i].opcode))
+            return fail(CLIFFT_E_UNSUPPORTED, "adjoint: program has measurement, noise or record opcode at " +
+                                                  std::to_string(i) + "; only noiseless measurement-free programs");
+    std::vector<int32_t> slot_of(n, -1);
+    std::vector<double> psign(std::max<uint32_t>(num_params, 1), 1.0);
+    for (uint32_t i = 0; i < num_params; ++i) {
+        const uint32_t pc = param_instr[i];
+        if (pc >= n) return fail(CLIFFT_E_INVALID, "adjoint: param_instr out of range");
+        const uint8_t op = p->host_bc[pc].opcode;
+        if (op != CLIFFT_OP_ARRAY_ROT && op != CLIFFT_OP_EXPAND_ROT)
+            return fail(CLIFFT_E_INVALID, "adjoint: parameter " + std::to_string(i) +
+                                              " is not an ARRAY_ROT / EXPAND_ROT instruction");
+        if (slot_of[pc] >= 0) return fail(CLIFFT_E_INVALID, "adjoint: two parameters on one instruction");
+        if (param_sign[i] != 1 && param_sign[i] != -1) return fail(CLIFFT_E_INVALID, "adjoint: param_sign must be +-1");
+        slot_of[pc] = (int32_t)i;
+        psign[i] = (double)param_sign[i];
+    }
+    auto* a = new clifft_svm_adjoint();
+    a->prog = p;
+    a->num_params = num_params;
+    const size_t o_ps = align_up(std::max<size_t>(n, 1) * 4, 256);
+    const size_t o_args = o_ps + align_up(psign.size() * 8, 256);
+    const size_t total = o_args + kAdjArgSlots * sizeof(AdjArgs);
+    hipError_t e = hipMalloc(&a->dev, total);
+    if (e == hipSuccess && n) e = hipMemcpy(a->dev, slot_of.data(), n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(static_cast<uint8_t*>(a->dev) + o_ps, psign.data(), psign.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        clifft_svm_adjoint_destroy(a);
+        return fail(CLIFFT_E_HIP, std::string("adjoint upload: ") + hipGetErrorString(e));
+    }
+    a->slot_of = static_cast<int32_t*>(a->dev);
+    a->psign = reinterpret_cast<double*>(static_cast<uint8_t*>(a->dev) + o_ps);
+    a->args_dev = reinterpret_cast<AdjArgs*>(static_cast<uint8_t*>(a->dev) + o_args);
+    // team shape: phi and lambda in LDS up to rank 12, else global slices
+    const uint32_t k = p->peak_rank;
+    const size_t red_dens = 16 * 4 * sizeof(double) + align_up((size_t)std::max<uint32_t>(p->num_ev, 1) * 8, 16);
+    a->global_amps = k > 12 || red_dens + ((size_t)32 << k) > 160 * 1024;
+    a->block = k <= 6 ? 64 : (k <= 8 ? 256 : (k <= 10 ? 512 : 1024));
+    a->lds_bytes = red_dens + (a->global_amps ? 0 : ((size_t)32 << k));
+    if (a->lds_bytes > 160 * 1024) {
+        clifft_svm_adjoint_destroy(a);
+        return fail(CLIFFT_E_UNSUPPORTED, "adjoint: too many EXP_VAL probes for the LDS norm table");
+    }
+    e = adjoint_occupancy(a->global_amps, a->block, a->lds_bytes, &a->per_cu);
+    if (e != hipSuccess) {
+        clifft_svm_adjoint_destroy(a);
+        return fail(CLIFFT_E_HIP, std::string("adjoint occupancy: ") + hipGetErrorString(e));
+    }
+    a->per_cu = std::max(a->per_cu, 1);
+    *out = a;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_adjoint_destroy(clifft_svm_adjoint* a) {
+    if (!a) return CLIFFT_OK;
+    (void)hipDeviceSynchronize();
+    if (a->dev) (void)hipFree(a->dev);
+    if (a->gscratch) (void)hipFree(a->gscratch);
+    if (a->staging) (void)hipFree(a->staging);
+    delete a;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_adjoint_run(clifft_svm_adjoint* a, const double* thetas, uint64_t batch, const double* coeffs,
+                           double* exp_vals, double* grads, uint32_t flags, void* stream) {
+    if (!a) return fail(CLIFFT_E_INVALID, "null adjoint");
+    if (batch == 0) return CLIFFT_OK;
+    if (a->num_params && !thetas) return fail(CLIFFT_E_INVALID, "null thetas");
+    if (grads && !coeffs && a->prog->num_ev) return fail(CLIFFT_E_INVALID, "gradients need coeffs");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    clifft_svm_program* p = a->prog;
+    const bool dev = (flags & CLIFFT_REC_DEVICE) != 0;
+    const size_t np = a->num_params, nev = p->num_ev;
+    const double *d_th = thetas, *d_c = coeffs;
+    double *d_ev = exp_vals, *d_g = grads;
+    if (!dev) {
+        const size_t s_th = align_up(batch * np * 8 + 8, 256), s_c = align_up(nev * 8 + 8, 256);
+        const size_t s_ev = align_up(batch * nev * 8 + 8, 256), s_g = align_up(batch * np * 8 + 8, 256);
+        int rc = adjoint_ensure(&a->staging, &a->staging_bytes, s_th + s_c + s_ev + s_g);
+        if (rc) return rc;
+        auto* b = static_cast<uint8_t*>(a->staging);
+        double* th = reinterpret_cast<double*>(b);
+        double* c = reinterpret_cast<double*>(b + s_th);
+        d_ev = exp_vals ? reinterpret_cast<double*>(b + s_th + s_c) : nullptr;
+        d_g = grads ? reinterpret_cast<double*>(b + s_th + s_c + s_ev) : nullptr;
+        if (np) HIP_TRY(hipMemcpyAsync(th, thetas, batch * np * 8, hipMemcpyHostToDevice, st));
+        if (coeffs && nev) HIP_TRY(hipMemcpyAsync(c, coeffs, nev * 8, hipMemcpyHostToDevice, st));
+        d_th = th;
+        d_c = coeffs ? c : nullptr;
+    }
+    DeviceInfo& di = device_info();
+    uint64_t grid = std::min<uint64_t>(batch, (uint64_t)a->per_cu * di.num_cus);
+    if (a->global_amps) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 8ull << 30;
+        const size_t per_team = (size_t)32 << p->peak_rank;
+        const size_t budget = std::min<size_t>((free_b + a->gscratch_bytes) / 2, 64ull << 30);
+        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, budget / per_team));
+        int rc = adjoint_ensure(&a->gscratch, &a->gscratch_bytes, grid * per_team);
+        if (rc) return rc;
+    }
+    AdjArgs A{};
+    A.thetas = d_th;
+    A.psign = a->psign;
+    A.slot_of = a->slot_of;
+    A.coeffs = d_c;
+    A.ev_out = d_ev;
+    A.grad_out = d_g;
+    A.gscratch = reinterpret_cast<dv2*>(a->gscratch);
+    A.batch = batch;
+    A.num_params = (uint32_t)np;
+    A.amp_elems = 1u << p->peak_rank;
+    AdjArgs* slot = a->args_dev + (a->args_slot++ % kAdjArgSlots);
+    HIP_TRY(hipMemcpyAsync(slot, &A, sizeof(A), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_adjoint(a->global_amps, (uint32_t)grid, a->block, a->lds_bytes, p->dp_dev, slot, st));
+    if (!dev) {
+        if (exp_vals && nev) HIP_TRY(hipMemcpyAsync(exp_vals, d_ev, batch * nev * 8, hipMemcpyDeviceToHost, st));
+        if (grads && np) HIP_TRY(hipMemcpyAsync(grads, d_g, batch * np * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
     return CLIFFT_OK;
 }
 

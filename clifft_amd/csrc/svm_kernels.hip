@@ -1,6 +1,8 @@
 // svm_kernels.hip -- team-interpreter kernels for gfx950 and their launchers.
 #include "svm_internal.h"
 #include "svm_interp.h"
+#include "svm_lane.h"
+#include "svm_adjoint.h"
 
 namespace csvm {
 
@@ -209,6 +211,101 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
     }
 }
 
+// Lane tier: one shot per lane.  Wavefront `wave` (0 <= wave < waves in the
+// workgroup) of this workgroup owns an LDS slice of lane_wave_bytes(); the
+// calling wavefront runs shot groups g = g_first, g_first + g_step, ...,
+// lane l taking shot s_lo + 64 g + l while that is below s_hi.
+__device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint64_t g_first, uint64_t g_step,
+                                           uint64_t s_lo, uint64_t s_hi, uint64_t seed) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    cptr<DevProgram> P = opaque(Pg);
+    const uint32_t peak = P->peak_rank, slots = P->total_meas_slots, ndet = P->num_det, nobs = P->num_obs;
+    const uint32_t wm = lane_words(slots), wd = lane_words(ndet), nwords = wm + wd + lane_words(nobs);
+    unsigned char* base = smem + (size_t)wave * lane_wave_bytes(peak, slots, ndet, nobs);
+    LaneInterp S;
+    S.amp = (lds_col_d2*)(base) + lane;
+    lds_col_u32* recs = (lds_col_u32*)(base + (64u * 16u << peak)) + lane;
+    S.rm = recs;
+    S.rd = recs + (wm << 6);
+    S.ro = S.rd + (wd << 6);
+    for (uint64_t g = g_first;; g += g_step) {
+        const uint64_t s0 = s_lo + g * 64u;
+        if (s0 >= s_hi) break;
+        const uint64_t s = s0 + lane;  // shot index within the launch
+        if (s < s_hi) {
+            cptr<ShotArgs> A = opaque(Ag);
+            P = opaque(Pg);
+            const uint32_t mode = A->mode;
+            for (uint32_t w = 0; w < nwords; ++w) recs[w << 6] = 0u;
+            S.px = S.pz = 0;
+            S.g_re = 1.0;
+            S.g_im = 0.0;
+            S.k = 0;
+            S.dust = 0;
+            S.discarded = 0;
+            S.reachable = 1;
+            S.logp = 0.0;
+            S.forced = nullptr;
+            S.rng = rng_for_shot(seed, A->shot_begin + s);
+            S.S(0, mk(1.0, 0.0));
+            S.next_noise = 0;
+            S.draw_next_noise(P->hazards, P->num_sites);
+            if (mode == MODE_FORCED) S.forced = A->forced_records + s * P->num_meas;
+            const uint32_t nev = P->num_ev;
+            double* ev_row = A->out_ev ? A->out_ev + s * nev : nullptr;
+            if (ev_row)
+                for (uint32_t i = 0; i < nev; ++i) ev_row[i] = 0.0;
+
+            S.run(Pg, ev_row);
+
+            A = opaque(Ag);
+            P = opaque(Pg);
+            const bool packed = (A->flags & CLIFFT_REC_PACKED) != 0;
+            const uint32_t n_exp = P->n_expected_obs;
+            cptr<uint8_t> expo = cp(P->expected_obs);
+            // record bits -> rows (bytes of 0/1, or little-endian bit-packed)
+            auto emit = [&](uint8_t* out, const lds_col_u32* src, uint32_t n, bool xor_expected) {
+                if (!out || n == 0) return;
+                if (packed) {
+                    const uint32_t nb = (n + 7u) / 8u;
+                    for (uint32_t j = 0; j < nb; ++j) {
+                        uint32_t byte = (src[(j >> 2) << 6] >> ((j & 3u) * 8u)) & 0xFFu;
+                        if (xor_expected) {
+                            uint32_t em = 0;
+                            for (uint32_t b = 0; b < 8; ++b)
+                                if (8 * j + b < n_exp) em |= (load_u8(expo, 8 * j + b) ? 1u : 0u) << b;
+                            byte ^= em;
+                        }
+                        if (8 * j + 8 > n) byte &= (1u << (n - 8 * j)) - 1u;
+                        out[s * nb + j] = (uint8_t)byte;
+                    }
+                } else {
+                    for (uint32_t i = 0; i < n; ++i) {
+                        uint32_t v = LaneInterp::rget(src, i);
+                        if (xor_expected && i < n_exp) v ^= load_u8(expo, i) ? 1u : 0u;
+                        out[s * n + i] = (uint8_t)v;
+                    }
+                }
+            };
+            emit(A->out_meas, S.rm, P->num_meas, false);
+            emit(A->out_det, S.rd, ndet, false);
+            emit(A->out_obs, S.ro, nobs, true);
+            if (A->out_disc) A->out_disc[s] = (uint8_t)S.discarded;
+            if (mode == MODE_FORCED && A->out_logp) A->out_logp[s] = S.reachable ? S.logp : kUnreachable;
+            if (S.dust && A->dust_counter) atomicAdd(A->dust_counter, (unsigned long long)S.dust);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256, 4) lane_kernel(const DevProgram* P, const ShotArgs* A) {
+    cptr<ShotArgs> Ag = cp(A);
+    const uint64_t wpb = blockDim.x >> 6;
+    const uint64_t wave_global = (uint64_t)blockIdx.x * wpb + uni(threadIdx.x >> 6);
+    lane_range(cp(P), Ag, wave_global, (uint64_t)gridDim.x * wpb, 0, Ag->shot_count, Ag->seed);
+}
+
 // One program: teams stride over the launch's shots.
 template <int FK, bool BT, bool GA>
 __global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram* P, const ShotArgs* A) {
@@ -224,6 +321,11 @@ __global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram*
 // One work item of the batched executor, kept out of line: the caller's
 // queue loop and the three team shapes then each get their own register
 // allocation (the save/restore at the call happens once per item).
+__device__ __noinline__ void batch_item_lane(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint32_t lo, uint32_t hi,
+                                            uint64_t seed) {
+    lane_range(Pg, Ag, uni(threadIdx.x >> 6), blockDim.x >> 6, lo, hi, seed);
+}
+
 template <int FK, bool BT, bool GA>
 __device__ __noinline__ void batch_item(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint32_t lo, uint32_t hi,
                                         uint32_t step, dv2* gamp, uint64_t seed) {
@@ -254,7 +356,9 @@ __global__ void __launch_bounds__(512, 4) batch_kernel(const BatchArgs* B) {
         cptr<DevProgram> Pg = cp(cp(Bc->progs)[pid]);
         cptr<ShotArgs> Ag = cp(Bc->args) + pid;
         const uint64_t seed = cp(Bc->seeds)[pid];
-        if (tier == TIER_WAVE) {
+        if (tier == TIER_LANE) {
+            batch_item_lane(Pg, Ag, lo, hi, seed);
+        } else if (tier == TIER_WAVE) {
             const uint32_t w = uni(threadIdx.x >> 6);
             batch_item<FK, false, false>(Pg, Ag, lo + w, hi, blockDim.x >> 6, nullptr, seed);
         } else if (tier == TIER_LDS) {
@@ -265,6 +369,64 @@ __global__ void __launch_bounds__(512, 4) batch_kernel(const BatchArgs* B) {
         }
         __syncthreads();
     }
+}
+
+// Expectation values and adjoint gradients: one workgroup team per theta
+// vector, phi and lambda in LDS (LDS tier) or in the workgroup's slice of
+// global scratch; LDS also holds the reduction scratch and the probe norms.
+template <bool GA>
+__global__ void __launch_bounds__(1024) adjoint_kernel(const DevProgram* P, const AdjArgs* A) {
+    typedef typename std::conditional<GA, glb_d2, lds_d2>::type AmpT;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    cptr<AdjArgs> Ag = cp(A);
+    AdjInterp<AmpT> S;
+    S.g.lane = threadIdx.x;
+    S.g.size = blockDim.x;
+    S.g.wave = uni(threadIdx.x >> 6);
+    S.g.nwaves = blockDim.x >> 6;
+    const uint32_t n = Ag->amp_elems;
+    unsigned char* p = smem;
+    if (GA) {
+        S.phi = (AmpT*)(Ag->gscratch + (size_t)blockIdx.x * 2u * n);
+    } else {
+        S.phi = (AmpT*)(p);
+        p += 2u * n * sizeof(dv2);
+    }
+    S.lam = S.phi + n;
+    S.red = (lds_f64*)(p);
+    p += 16 * 4 * sizeof(double);
+    lds_f64* dens = (lds_f64*)(p);
+    const uint64_t batch = Ag->batch;
+    for (uint64_t b = blockIdx.x; b < batch; b += gridDim.x) {
+        cptr<AdjArgs> A2 = opaque(Ag);
+        cptr<DevProgram> P0 = cp(P);
+        const uint32_t np = A2->num_params, nev = P0->num_ev;
+        const double* th = A2->thetas + b * np;
+        double* ev_row = A2->ev_out ? A2->ev_out + b * nev : nullptr;
+        double* grad_row = A2->grad_out ? A2->grad_out + b * np : nullptr;
+        if (grad_row && S.g.lane == 0)
+            for (uint32_t i = 0; i < np; ++i) grad_row[i] = 0.0;
+        S.forward(cp(P), Ag, th, ev_row, dens);
+        if (grad_row) S.backward(cp(P), Ag, th, dens, grad_row);
+        team_sync<true>();
+    }
+}
+
+hipError_t adjoint_occupancy(bool global_amps, uint32_t block, size_t lds_bytes, int* blocks_per_cu) {
+    const void* fn = global_amps ? reinterpret_cast<const void*>(adjoint_kernel<true>)
+                                 : reinterpret_cast<const void*>(adjoint_kernel<false>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, (int)block, lds_bytes);
+}
+
+hipError_t launch_adjoint(bool global_amps, uint32_t grid, uint32_t block, size_t lds_bytes, const DevProgram* P,
+                          const AdjArgs* A, hipStream_t st) {
+    if (global_amps)
+        hipLaunchKernelGGL(adjoint_kernel<true>, dim3(grid), dim3(block), lds_bytes, st, P, A);
+    else
+        hipLaunchKernelGGL(adjoint_kernel<false>, dim3(grid), dim3(block), lds_bytes, st, P, A);
+    return hipGetLastError();
 }
 
 __global__ void clear_u32_kernel(uint32_t* p, uint32_t n) {
@@ -289,6 +451,18 @@ static hipError_t launch_one(const LaunchShape& L, const DevProgram* P, const Sh
     return hipGetLastError();
 }
 
+static hipError_t launch_lane(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
+    static bool attr_done = false;
+    if (!attr_done) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(lane_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_done = true;
+    }
+    hipLaunchKernelGGL(lane_kernel, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
+    return hipGetLastError();
+}
+
 template <int FK>
 static hipError_t launch_fk(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
     if (L.block_team)
@@ -296,7 +470,25 @@ static hipError_t launch_fk(const LaunchShape& L, const DevProgram* P, const Sho
     return L.global_amps ? launch_one<FK, false, true>(L, P, A, st) : launch_one<FK, false, false>(L, P, A, st);
 }
 
+template <int FK>
+static const void* shot_fn_fk(const LaunchShape& L) {
+    if (L.block_team)
+        return L.global_amps ? reinterpret_cast<const void*>(shot_kernel<FK, true, true>)
+                             : reinterpret_cast<const void*>(shot_kernel<FK, true, false>);
+    return L.global_amps ? reinterpret_cast<const void*>(shot_kernel<FK, false, true>)
+                         : reinterpret_cast<const void*>(shot_kernel<FK, false, false>);
+}
+
+hipError_t shot_occupancy(const LaunchShape& L, int* blocks_per_cu) {
+    const void* fn = L.lane ? reinterpret_cast<const void*>(lane_kernel)
+                            : (L.nw == 1 ? shot_fn_fk<FRAME_W1>(L) : shot_fn_fk<FRAME_LANES>(L));
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, (int)L.block, L.lds_bytes);
+}
+
 hipError_t launch_shots(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
+    if (L.lane) return launch_lane(L, P, A, st);
     switch (L.nw) {
         case 1: return launch_fk<FRAME_W1>(L, P, A, st);
         default: return launch_fk<FRAME_LANES>(L, P, A, st);
@@ -312,7 +504,8 @@ hipError_t batch_prepare(int nw) {
     static bool done[2] = {false, false};
     const void* fn = nw == 1 ? batch_fn<FRAME_W1>() : batch_fn<FRAME_LANES>();
     if (done[nw == 1]) return hipSuccess;
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    // the queue head is 16 B of static LDS; dynamic LDS may use the rest
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kBatchMaxDynLds);
     if (e == hipSuccess) done[nw == 1] = true;
     return e;
 }

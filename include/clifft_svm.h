@@ -335,9 +335,10 @@ int clifft_svm_execute(clifft_svm_program* prog, clifft_svm_state* state);
  * bytecode length run in ONE persistent launch: work items (a shot range of
  * one program) are sorted by descending estimated cost and drained by the
  * workgroups from a device queue.  Each item runs with the team shape of
- * its tier: one shot per wavefront (peak_rank <= wave_rank_max), one shot
- * per workgroup with LDS-resident amplitudes (<= lds_rank_max), or one shot
- * per workgroup with amplitudes in a per-workgroup global slice.
+ * its tier: one shot per lane (peak_rank <= lane_rank_max, at most 64
+ * qubits), one shot per wavefront (peak_rank <= wave_rank_max), one shot per
+ * workgroup with LDS-resident amplitudes (<= lds_rank_max), or one shot per
+ * workgroup with amplitudes in a per-workgroup global slice.
  *
  * Shot s of program i draws from the per-shot stream of (seeds[i], s), as in
  * clifft_svm_sample, so the records equal that call's; the team shape (and
@@ -354,6 +355,9 @@ typedef struct {
     uint32_t wave_chunk;    /* shots per work item, wavefront tier */
     uint32_t block_chunk;   /* shots per work item, workgroup tiers */
     uint32_t grid;          /* workgroups (0 = resident capacity of the device) */
+    int32_t lane_rank_max;  /* <= 64-qubit programs up to this rank run one shot
+                               per lane (-1 = never), <= 5 */
+    uint32_t lane_chunk;    /* shots per work item, lane tier */
 } clifft_svm_batch_tuning;
 
 /* outs[i] are device pointers; CLIFFT_REC_PACKED is honoured. */
@@ -374,6 +378,30 @@ int clifft_svm_batch_run(clifft_svm_batch* batch, const uint64_t* seeds, void* s
 /* Same work through a captured HIP graph (queue reset + persistent launch),
  * captured on first use per stream and tuning and replayed afterwards. */
 int clifft_svm_batch_run_graph(clifft_svm_batch* batch, const uint64_t* seeds, void* stream);
+
+/* --- differentiation ---------------------------------------------------------*/
+/* No counterpart in the reference (it has no gradients; its EXP_VAL probe is
+ * src/clifft/svm/svm_kernels.inl:2031-2173).  For a noiseless,
+ * measurement-free program whose parameterised Z rotations compiled to
+ * ARRAY_ROT / EXPAND_ROT instructions (param_instr[i], with phase
+ * exp(i param_sign[i] theta_i)), compute for a batch of theta vectors the
+ * EXP_VAL results and the gradient of F = sum_j coeffs[j] * exp_vals[j] with
+ * respect to theta, by one forward and one backward (adjoint) sweep per
+ * theta vector on the device. */
+typedef struct clifft_svm_adjoint clifft_svm_adjoint;
+
+int clifft_svm_adjoint_create(clifft_svm_program* prog, const uint32_t* param_instr,
+                              const int8_t* param_sign, uint32_t num_params,
+                              clifft_svm_adjoint** out);
+int clifft_svm_adjoint_destroy(clifft_svm_adjoint* adj);
+/* thetas [batch x num_params]; coeffs [num_exp_vals] (needed with grads);
+ * exp_vals [batch x num_exp_vals] or NULL; grads [batch x num_params] or NULL
+ * (forward only).  flags: CLIFFT_REC_DEVICE = every pointer is device memory
+ * and the call is asynchronous on `stream`; otherwise host memory and the
+ * call returns with the results written. */
+int clifft_svm_adjoint_run(clifft_svm_adjoint* adj, const double* thetas, uint64_t batch,
+                           const double* coeffs, double* exp_vals, double* grads, uint32_t flags,
+                           void* stream);
 
 /* One-call form.  Without CLIFFT_BATCH_PER_PROGRAM it builds a batch (with
  * the default tuning), runs it once and destroys it; outputs may be host

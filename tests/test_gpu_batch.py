@@ -34,8 +34,8 @@ def mods():
     return svm, batch
 
 
-# peaks spanning every tier: wavefront teams, LDS workgroup teams, global
-PEAKS = [2, 4, 6, 7, 8, 9, 11, 12, 13, 14, 15]
+# peaks spanning every tier: lanes, wavefront teams, LDS workgroup teams, global
+PEAKS = [0, 2, 3, 4, 6, 7, 8, 9, 11, 12, 13, 14, 15]
 
 
 def mixed_programs(seed, count, wide=False):
@@ -108,7 +108,9 @@ def _snapshot(bufs):
 
 TUNINGS = [dict(block_threads=512), dict(lds_rank_max=9), dict(lds_rank_max=13), dict(wave_rank_max=5),
            dict(wave_rank_max=8, lds_rank_max=8), dict(wave_chunk=4, block_chunk=1), dict(grid=7),
-           dict(block_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32)]
+           dict(block_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32),
+           dict(lane_rank_max=-1), dict(lane_rank_max=5, lane_chunk=100), dict(lane_rank_max=1, lane_chunk=64),
+           dict(block_threads=512, lane_rank_max=4, lane_chunk=3000)]
 
 
 def test_batch_tunings_agree(mods):
@@ -196,5 +198,7 @@ def test_batch_rejects_bad_tuning(mods):
         b.set_tuning(wave_rank_max=9)
     with pytest.raises(SvmError):
         b.set_tuning(lds_rank_max=14)
+    with pytest.raises(SvmError):
+        b.set_tuning(lane_rank_max=6)
     b.run(seeds)  # still usable with the previous tuning
     b.close()

@@ -34,9 +34,10 @@ def svm():
     return svm
 
 
-# (num_qubits, peak): wave-team LDS, workgroup-team LDS, global-memory tier,
-# and a >64-qubit frame held in LDS
-SHAPES = [(8, 3), (12, 6), (16, 8), (20, 10), (24, 12), (24, 13), (24, 14), (30, 15), (100, 6), (130, 9)]
+# (num_qubits, peak): one shot per lane (peak <= 3, <= 64 qubits), wave-team
+# LDS, workgroup-team LDS, global-memory tier, and >64-qubit frames
+SHAPES = [(5, 0), (6, 1), (10, 2), (8, 3), (64, 3), (12, 6), (16, 8), (20, 10), (24, 12), (24, 13), (24, 14),
+          (30, 15), (100, 3), (100, 6), (130, 9)]
 
 
 @pytest.mark.parametrize("nq,peak", SHAPES)
@@ -53,7 +54,7 @@ def test_sample_bit_exact(svm, orc, nq, peak, seed):
     np.testing.assert_allclose(res.exp_vals, e, atol=ATOL, rtol=0)
 
 
-@pytest.mark.parametrize("peak", [4, 9, 14])
+@pytest.mark.parametrize("peak", [2, 4, 9, 14])
 def test_packed_records(svm, peak):
     announce("packed", peak)
     p = random_program(77 + peak, num_qubits=20, peak=peak, length=200)
@@ -63,7 +64,7 @@ def test_packed_records(svm, peak):
         assert np.array_equal(np.packbits(x, axis=1, bitorder="little"), y)
 
 
-@pytest.mark.parametrize("nq,peak", [(8, 4), (12, 7), (16, 10), (20, 14)])
+@pytest.mark.parametrize("nq,peak", [(6, 2), (9, 3), (8, 4), (12, 7), (16, 10), (20, 14)])
 def test_record_log_probs(svm, orc, nq, peak):
     announce("forced", nq, peak)
     p = random_measure_only_program(peak, num_qubits=nq, peak=peak, length=120)
@@ -107,9 +108,10 @@ def test_execute_preset_state(svm, orc, nq, peak, seed):
     assert list(map(int, g.rng)) == list(map(int, o.rng))
 
 
-def test_survivors_counts(svm, orc):
-    announce("survivors")
-    p = random_program(5, num_qubits=12, peak=5, length=250, postselect=True)
+@pytest.mark.parametrize("peak", [2, 5])
+def test_survivors_counts(svm, orc, peak):
+    announce("survivors", peak)
+    p = random_program(5, num_qubits=12, peak=peak, length=250, postselect=True)
     if not p.has_postselection:
         pytest.skip("no postselect drawn")
     g = svm.sample_survivors(p, 500, seed=11, keep_records=True)
@@ -121,11 +123,12 @@ def test_survivors_counts(svm, orc):
     assert np.array_equal(g.detectors, r["detectors"])
 
 
-def test_reference_stream_first_shot(svm, orc):
+@pytest.mark.parametrize("peak", [3, 5])
+def test_reference_stream_first_shot(svm, orc, peak):
     """Shot 0 of the device equals the first shot of the reference's single
     stream (svm.cc:210-226)."""
-    announce("first-shot")
-    p = random_program(8, num_qubits=10, peak=5, length=200)
+    announce("first-shot", peak)
+    p = random_program(8, num_qubits=10, peak=peak, length=200)
     m, d, o, e, _ = orc.sample(p, 1, seed=1234, per_shot=False)
     res = svm.sample(p, 1, seed=1234)
     assert np.array_equal(res.measurements, m) and np.array_equal(res.detectors, d)
