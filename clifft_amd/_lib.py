@@ -73,6 +73,13 @@ SIGNATURES = {
     "clifft_svm_sample_survivors": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                                    ctypes.c_uint32, ctypes.POINTER(Survivors),
                                                    ctypes.POINTER(Records), ctypes.c_void_p]),
+    "clifft_svm_sample_k": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                           ctypes.c_uint32, ctypes.POINTER(Records), ctypes.c_void_p]),
+    "clifft_svm_sample_k_survivors": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                     ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(Survivors),
+                                                     ctypes.POINTER(Records), ctypes.c_void_p]),
+    "clifft_svm_noise_site_probabilities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                           ctypes.POINTER(ctypes.c_uint32)]),
     "clifft_svm_record_log_probs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                                    ctypes.c_void_p, ctypes.c_void_p]),
     "clifft_svm_execute": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(State)]),
@@ -98,11 +105,22 @@ SIGNATURES = {
 
 
 def lib():
-    """Load the native library (raises if it is not built)."""
+    """Load the native library (raises if it is not built).
+
+    torch is imported first when it is installed: the HIP runtime it ships
+    and the system one share the soname libamdhip64.so.7, so loading torch's
+    first makes this library bind to the same runtime.  Tensors, streams and
+    events then mean the same thing on both sides; loaded the other way
+    round, the process would hold two HIP runtimes and torch would see no
+    device."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise SvmError(f"{LIB_PATH} is not built; run `python -m clifft_amd.build`")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -91,7 +91,21 @@ struct PresetState {
     uint64_t pz[16];
 };
 
-enum : uint32_t { MODE_SAMPLE = 0, MODE_FORCED = 1, MODE_PRESET = 2 };
+enum : uint32_t { MODE_SAMPLE = 0, MODE_FORCED = 1, MODE_PRESET = 2, MODE_KFAULT = 3 };
+
+// Importance sampling with exactly k faults per shot: the fault-set law over
+// the n = num_sites + num_readout sites (svm.cc:356-546).  Uniform mode
+// draws a partial Fisher-Yates prefix; otherwise a walk over the table
+// dp[i * (k + 1) + j] = e_j(w[i..n)) with odds ratios w.
+constexpr uint32_t kMaxForcedFaults = 64;
+struct KfTables {
+    const double* w;   // [n]
+    const double* dp;  // [(n + 1) x (k + 1)]
+    uint32_t n;        // sites + readout entries
+    uint32_t n_q;      // quantum noise sites (indices below n_q)
+    uint32_t k;
+    uint32_t uniform;
+};
 
 struct ShotArgs {
     uint64_t shot_begin;  // global index of the first shot of this launch
@@ -114,7 +128,14 @@ struct ShotArgs {
     uint8_t* preset_records;          // MODE_PRESET meas|det|obs bytes
     double* preset_ev;                // MODE_PRESET exp_vals
     unsigned long long* dust_counter; // optional telemetry
+    const KfTables* kf;               // MODE_KFAULT
 };
+
+// Team-tier LDS scratch of a workgroup: the reduction area (16 waves x 4
+// doubles) and, per wavefront, the phases of pending fused expansions
+// (kMaxFusedExpand (re, im) pairs, padded to 16 doubles).
+constexpr uint32_t kMaxFusedExpand = 5;
+constexpr uint32_t kTeamScratchBytes = 16 * 4 * 8 + 16 * 16 * 8;
 
 // Lane tier (one shot per lane): words of bit-packed records per lane, and
 // the LDS bytes of one wavefront: 2^peak amplitude rows of 64 x 16 B, then

@@ -291,6 +291,13 @@ struct clifft_svm_program {
     bool has_forced_ops = false;  // bytecode already contains *_FORCED opcodes
     uint64_t num_array_ops = 0;
     double bytes_per_shot = 0.0;
+    // importance sampling: per-site fault probabilities (quantum sites, then
+    // readout entries) and the device tables of the last k used
+    std::vector<double> fault_probs;
+    uint32_t num_sites = 0;
+    void* kf_dev = nullptr;
+    size_t kf_dev_bytes = 0;
+    int64_t kf_k = -1;
     // launch planning
     LaunchShape base_shape;
     // staging for host outputs (grown on demand)
@@ -309,8 +316,11 @@ int frame_nw(uint32_t words) {
     return words <= 1 ? 1 : 0;  // 0 = frame spread over the wavefront's lanes
 }
 
-size_t rec_stride_of(const clifft_svm_program* p) {
-    return align_up(std::max<size_t>(p->total_slots + p->num_det + p->num_obs, 4), 16);
+// LDS bytes of one wavefront's record block (+ the k-fault scratch of
+// 5 * kMaxForcedFaults words at its end when sampling with forced faults).
+size_t rec_stride_of(const clifft_svm_program* p, bool kfault = false) {
+    return align_up(std::max<size_t>(p->total_slots + p->num_det + p->num_obs, 4), 16) +
+           (kfault ? 20u * kMaxForcedFaults : 0u);
 }
 
 // Highest peak rank run one shot per lane (CLIFFT_LANE_MAX_RANK overrides;
@@ -324,7 +334,7 @@ int lane_max_rank() {
 }
 
 // Choose the team shape and storage tier for a program's peak rank.
-LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true) {
+LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool kfault = false) {
     LaunchShape L;
     L.nw = (int)p->nw;
     const uint32_t k = p->peak_rank;
@@ -339,9 +349,9 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true) {
             return L;
         }
     }
-    const size_t rec_stride = rec_stride_of(p);
+    const size_t rec_stride = rec_stride_of(p, kfault);
     const size_t amp_bytes = (size_t)16 << k;
-    const size_t red = 16 * 4 * sizeof(double);
+    const size_t red = kTeamScratchBytes;
     if (k <= 7) {
         L.block_team = false;
         L.block = 256;
@@ -432,9 +442,9 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStr
     return CLIFFT_OK;
 }
 
-void fill_common_args(clifft_svm_program* p, ShotArgs& A, void* gscratch) {
+void fill_common_args(clifft_svm_program* p, ShotArgs& A, void* gscratch, bool kfault = false) {
     std::memset(&A, 0, sizeof(A));
-    A.rec_stride = (uint32_t)rec_stride_of(p);
+    A.rec_stride = (uint32_t)rec_stride_of(p, kfault);
     A.amp_elems = 1u << p->peak_rank;
     A.gscratch = reinterpret_cast<dv2*>(gscratch);
     A.dust_counter = p->dust_dev;
@@ -493,6 +503,13 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     p->has_forced_ops = has_forced;
     p->host_bc.assign(view->bytecode, view->bytecode + view->num_instructions);
     p->khist = khist;
+    p->num_sites = view->num_sites;
+    for (uint32_t s = 0; s < view->num_sites; ++s) {  // svm.cc:356-369
+        double acc = 0.0;
+        for (uint32_t c = view->site_chan_off[s]; c < view->site_chan_off[s + 1]; ++c) acc += view->chan_prob[c];
+        p->fault_probs.push_back(acc);
+    }
+    for (uint32_t e = 0; e < view->num_readout; ++e) p->fault_probs.push_back(view->ro_prob[e]);
     if (view->expected_observables)
         p->expected_obs.assign(view->expected_observables, view->expected_observables + view->num_expected_observables);
     const uint32_t fbits = view->words * 64;
@@ -624,6 +641,7 @@ int clifft_svm_program_destroy(clifft_svm_program* p) {
     if (p->dev_blob) (void)hipFree(p->dev_blob);
     if (p->out_dev) (void)hipFree(p->out_dev);
     if (p->dust_dev) (void)hipFree(p->dust_dev);
+    if (p->kf_dev) (void)hipFree(p->kf_dev);
     if (p->args_dev) (void)hipFree(p->args_dev);
     delete p;
     return CLIFFT_OK;
@@ -645,7 +663,7 @@ int clifft_svm_program_info_get(const clifft_svm_program* p, clifft_svm_program_
 // copied to the caller's host arrays).
 static int run_shots(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags, uint32_t mode,
                      const clifft_svm_records* out, const uint8_t* forced_host, double* logp_host,
-                     hipStream_t st) {
+                     hipStream_t st, const KfTables* kf = nullptr) {
     const bool dev_out = (flags & CLIFFT_REC_DEVICE) != 0;
     const bool packed = (flags & CLIFFT_REC_PACKED) != 0;
     if (mode == MODE_SAMPLE && p->has_forced_ops)
@@ -653,12 +671,14 @@ static int run_shots(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint3
     const uint64_t chunk_max = dev_out ? shots : (uint64_t)1 << 20;
     for (uint64_t base = 0; base < shots; base += chunk_max) {
         const uint64_t n = std::min(chunk_max, shots - base);
-        LaunchShape L = p->base_shape;
+        const bool kfault = mode == MODE_KFAULT;
+        LaunchShape L = kfault ? plan_shape(p, false, true) : p->base_shape;
         void* gs = nullptr;
         int rc = finalize_shape(p, L, n, st, &gs);
         if (rc) return rc;
         ShotArgs A;
-        fill_common_args(p, A, gs);
+        fill_common_args(p, A, gs, kfault);
+        A.kf = kf;
         A.shot_begin = base;
         A.shot_count = n;
         A.seed = seed;
@@ -724,8 +744,11 @@ int clifft_svm_sample(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint
     return CLIFFT_OK;
 }
 
-int clifft_svm_sample_survivors(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags,
-                                clifft_svm_survivors* summary, const clifft_svm_records* out, void* stream) {
+}  // extern "C"
+
+static int survivors_impl(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags,
+                          clifft_svm_survivors* summary, const clifft_svm_records* out, void* stream, uint32_t mode,
+                          const KfTables* kf) {
     if (!p || !summary) return fail(CLIFFT_E_INVALID, "null argument");
     if (flags & CLIFFT_REC_DEVICE) return fail(CLIFFT_E_UNSUPPORTED, "survivor compaction writes host memory");
     summary->total_shots = shots;
@@ -751,7 +774,7 @@ int clifft_svm_sample_survivors(clifft_svm_program* p, uint64_t shots, uint64_t 
     }
     // observables are always fetched unpacked for counting
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
-    int rc = run_shots(p, shots, seed, flags & ~CLIFFT_REC_PACKED, MODE_SAMPLE, &r, nullptr, nullptr, st);
+    int rc = run_shots(p, shots, seed, flags & ~CLIFFT_REC_PACKED, mode, &r, nullptr, nullptr, st, kf);
     if (rc) return rc;
     const size_t bm = row_bytes(p->num_meas, false), bd = row_bytes(p->num_det, false);
     const size_t pm = row_bytes(p->num_meas, packed), pd = row_bytes(p->num_det, packed), po = row_bytes(p->num_obs, packed);
@@ -788,6 +811,117 @@ int clifft_svm_sample_survivors(clifft_svm_program* p, uint64_t shots, uint64_t 
         row++;
     }
     summary->passed_shots = row;
+    return CLIFFT_OK;
+}
+
+extern "C" {
+
+int clifft_svm_sample_survivors(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags,
+                                clifft_svm_survivors* summary, const clifft_svm_records* out, void* stream) {
+    return survivors_impl(p, shots, seed, flags, summary, out, stream, MODE_SAMPLE, nullptr);
+}
+
+// Device tables of the k-fault law (svm.cc:376-431, 525-546) for this k.
+static int kfault_tables(clifft_svm_program* p, uint32_t k, hipStream_t st, const KfTables** out) {
+    const std::vector<double>& probs = p->fault_probs;
+    const uint32_t n = (uint32_t)probs.size();
+    if (k > n)
+        return fail(CLIFFT_E_INVALID, "k (" + std::to_string(k) + ") exceeds total fault sites (" + std::to_string(n) + ")");
+    uint32_t certain = 0, impossible = 0;
+    for (double q : probs) {
+        if (q <= 0.0)
+            impossible++;
+        else if (q >= 1.0)
+            certain++;
+    }
+    if (k < certain || k > n - impossible)
+        return fail(CLIFFT_E_INVALID, "k-fault stratum k=" + std::to_string(k) + " has zero probability mass (" +
+                                          std::to_string(certain) + " sites have p=1, " + std::to_string(impossible) +
+                                          " sites have p=0)");
+    if (k > kMaxForcedFaults)
+        return fail(CLIFFT_E_UNSUPPORTED, "device k-fault sampling supports k <= " + std::to_string(kMaxForcedFaults));
+    bool uniform = true;
+    for (uint32_t i = 1; i < n; ++i)
+        if (probs[i] != probs[0]) uniform = false;
+    const size_t o_w = align_up(sizeof(KfTables), 256), o_dp = o_w + align_up((size_t)(n + 1) * 8, 256);
+    const size_t total = o_dp + (size_t)(n + 1) * (k + 1) * 8;
+    if ((int64_t)k != p->kf_k || total > p->kf_dev_bytes) {
+        if (total > p->kf_dev_bytes) {
+            if (p->kf_dev) {
+                HIP_TRY(hipDeviceSynchronize());
+                (void)hipFree(p->kf_dev);
+            }
+            p->kf_dev = nullptr;
+            p->kf_dev_bytes = 0;
+            HIP_TRY(hipMalloc(&p->kf_dev, total));
+            p->kf_dev_bytes = total;
+        }
+        std::vector<uint8_t> staging(total, 0);
+        auto* w = reinterpret_cast<double*>(staging.data() + o_w);
+        auto* dp = reinterpret_cast<double*>(staging.data() + o_dp);
+        if (!uniform) {
+            double sum = 0.0;  // odds ratios rescaled to mean 1
+            for (uint32_t i = 0; i < n; ++i) {
+                const double q = std::clamp(probs[i], 0.0, 1.0 - 1e-15);
+                w[i] = q / (1.0 - q);
+                sum += w[i];
+            }
+            if (sum > 0.0)
+                for (uint32_t i = 0; i < n; ++i) w[i] *= (double)n / sum;
+            const uint32_t stride = k + 1;  // dp[i][j] = e_j(w[i..n))
+            for (uint32_t i = 0; i <= n; ++i) dp[(size_t)i * stride] = 1.0;
+            for (int i = (int)n - 1; i >= 0; --i) {
+                const uint32_t max_j = std::min(n - (uint32_t)i, k);
+                for (uint32_t j = 1; j <= max_j; ++j)
+                    dp[(size_t)i * stride + j] =
+                        dp[(size_t)(i + 1) * stride + j] + w[i] * dp[(size_t)(i + 1) * stride + j - 1];
+            }
+        }
+        auto* base = static_cast<uint8_t*>(p->kf_dev);
+        KfTables T{};
+        T.w = reinterpret_cast<const double*>(base + o_w);
+        T.dp = reinterpret_cast<const double*>(base + o_dp);
+        T.n = n;
+        T.n_q = p->num_sites;
+        T.k = k;
+        T.uniform = uniform ? 1u : 0u;
+        std::memcpy(staging.data(), &T, sizeof(T));
+        HIP_TRY(hipMemcpyAsync(p->kf_dev, staging.data(), total, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));  // staging is released on return
+        p->kf_k = k;
+    }
+    *out = static_cast<const KfTables*>(p->kf_dev);
+    return CLIFFT_OK;
+}
+
+int clifft_svm_sample_k(clifft_svm_program* p, uint64_t shots, uint32_t k, uint64_t seed, uint32_t flags,
+                        const clifft_svm_records* out, void* stream) {
+    if (!p) return fail(CLIFFT_E_INVALID, "null program");
+    if (shots == 0) return CLIFFT_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    const KfTables* kf = nullptr;
+    int rc = kfault_tables(p, k, st, &kf);
+    if (rc) return rc;
+    rc = run_shots(p, shots, seed, flags, MODE_KFAULT, out, nullptr, nullptr, st, kf);
+    if (rc) return rc;
+    if (flags & CLIFFT_REC_DEVICE) HIP_TRY(hipStreamSynchronize(st));
+    return CLIFFT_OK;
+}
+
+int clifft_svm_sample_k_survivors(clifft_svm_program* p, uint64_t shots, uint32_t k, uint64_t seed, uint32_t flags,
+                                  clifft_svm_survivors* summary, const clifft_svm_records* out, void* stream) {
+    if (!p || !summary) return fail(CLIFFT_E_INVALID, "null argument");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    const KfTables* kf = nullptr;
+    int rc = kfault_tables(p, k, st, &kf);
+    if (rc) return rc;
+    return survivors_impl(p, shots, seed, flags, summary, out, stream, MODE_KFAULT, kf);
+}
+
+int clifft_svm_noise_site_probabilities(const clifft_svm_program* p, double* out, uint32_t* count) {
+    if (!p || !count) return fail(CLIFFT_E_INVALID, "null argument");
+    if (out) std::memcpy(out, p->fault_probs.data(), p->fault_probs.size() * sizeof(double));
+    *count = (uint32_t)p->fault_probs.size();
     return CLIFFT_OK;
 }
 
@@ -934,7 +1068,7 @@ uint32_t tier_of(const clifft_svm_program* p, const clifft_svm_batch_tuning& t) 
 size_t lds_need(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
     const size_t waves = block / 64;
     if (tier == TIER_LANE) return waves * lane_wave_bytes(p->peak_rank, p->total_slots, p->num_det, p->num_obs);
-    size_t lds = 16 * 4 * sizeof(double) + waves * rec_stride_of(p);
+    size_t lds = kTeamScratchBytes + waves * rec_stride_of(p);
     if (tier == TIER_WAVE) lds += waves * ((size_t)16 << p->peak_rank);
     if (tier == TIER_LDS) lds += (size_t)16 << p->peak_rank;
     return lds;
@@ -971,7 +1105,7 @@ int batch_plan(clifft_svm_batch* b, uint64_t cap, hipStream_t st) {
     if (rc) return rc;
     DeviceInfo& di = device_info();
     const uint32_t block = t.block_threads;
-    size_t lds = 16 * 4 * sizeof(double);
+    size_t lds = kTeamScratchBytes;
     uint32_t kglobal = 0;
     bool any_global = false;
     struct Pending {

@@ -90,6 +90,9 @@ struct Interp {
     TeamGeom g;
     AmpT* amp;
     lds_f64* red;
+    lds_f64* pend;       // this wavefront's pending-expansion phases
+    uint32_t npend;      // expansions recorded but not yet written
+    uint32_t pend_mask;  // which of them carry a phase
     __attribute__((address_space(3))) uint8_t* meas;
     __attribute__((address_space(3))) uint8_t* det;
     __attribute__((address_space(3))) uint8_t* obs;
@@ -103,6 +106,11 @@ struct Interp {
     double logp;
     uint32_t dust;
     cptr<uint8_t> forced;
+    // MODE_KFAULT: the shot's forced noise sites then readout entries, each
+    // ascending, in this wavefront's LDS block (identical in every wave)
+    uint32_t kf;
+    const __attribute__((address_space(3))) uint32_t* kf_list;
+    uint32_t kf_nn, kf_npos, kf_nr, kf_rpos;
 
     // ----------------------------------------------------------------- gamma
     CSVM_INLINE void mul_phase(double re, double im) {
@@ -114,12 +122,31 @@ struct Interp {
         g_re = -g_re;
         g_im = -g_im;
     }
+    // Write the pending expansions (ranks k - npend .. k).
+    CSVM_INLINE void flush_expand() {
+        if (npend == 0) return;
+        sweep_expand_multi<BT>(amp, k - npend, npend, pend, pend_mask, g);
+        npend = 0;
+        pend_mask = 0;
+    }
+    // Opcodes that never read or write amplitudes nor depend on them, so
+    // pending expansions may stay pending across them.
+    static constexpr uint64_t kNoAmpOps =
+        (1ull << CLIFFT_OP_FRAME_CNOT) | (1ull << CLIFFT_OP_FRAME_CZ) | (1ull << CLIFFT_OP_FRAME_H) |
+        (1ull << CLIFFT_OP_FRAME_S) | (1ull << CLIFFT_OP_FRAME_S_DAG) | (1ull << CLIFFT_OP_FRAME_SWAP) |
+        (1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) | (1ull << CLIFFT_OP_EXPAND_T_DAG) |
+        (1ull << CLIFFT_OP_EXPAND_ROT) | (1ull << CLIFFT_OP_MEAS_DORMANT_STATIC) |
+        (1ull << CLIFFT_OP_MEAS_DORMANT_RANDOM) | (1ull << CLIFFT_OP_APPLY_PAULI) | (1ull << CLIFFT_OP_NOISE) |
+        (1ull << CLIFFT_OP_NOISE_BLOCK) | (1ull << CLIFFT_OP_READOUT_NOISE) | (1ull << CLIFFT_OP_DETECTOR) |
+        (1ull << CLIFFT_OP_OBSERVABLE);
+
     // svm.h:177-186
     CSVM_INLINE void scale_magnitude(double s) {
         g_re *= s;
         g_im *= s;
         double gm = hypot(g_re, g_im);
         if (gm > 1e100 || (gm < 1e-100 && gm > 0.0)) {
+            flush_expand();
             sweep_scale<BT>(amp, k, gm, g);
             g_re /= gm;
             g_im /= gm;
@@ -247,8 +274,13 @@ struct Interp {
                 break;
             }
         }
-        next_noise++;
-        draw_next_noise(P);
+        if (kf) {  // svm.h:218-225
+            next_noise = kf_npos < kf_nn ? uni(kf_list[kf_npos]) : kNoNoise;
+            kf_npos++;
+        } else {
+            next_noise++;
+            draw_next_noise(P);
+        }
     }
     CSVM_INLINE uint32_t list_parity(cptr<uint32_t> off, cptr<uint32_t> tgt, uint32_t l) {
         uint32_t par = 0;
@@ -370,6 +402,105 @@ struct Interp {
         return res;
     }
 
+    // ------------------------------------------------- k-fault fault sets
+    // Choose this shot's fault set (svm.cc:435-520) into `list` (per-wave
+    // LDS, room for 5 * kMaxForcedFaults words: picks, then the sparse
+    // Fisher-Yates map of at most 2k positions) and arm the noise cursor.
+    // Every lane runs the same uniform loop and draws; lane 0 stores.
+    CSVM_INLINE void kfault_prepare(cptr<KfTables> T, __attribute__((address_space(3))) uint32_t* list) {
+        const uint32_t n = T->n, k = T->k, n_q = T->n_q;
+        const bool lead = (threadIdx.x & 63u) == 0;
+        uint32_t m = 0;
+        if (T->uniform) {
+            __attribute__((address_space(3))) uint32_t* keys = list + kMaxForcedFaults;
+            __attribute__((address_space(3))) uint32_t* vals = keys + 2 * kMaxForcedFaults;
+            uint32_t nk = 0;
+            auto find = [&](uint32_t pos) -> uint32_t {  // index in the map, or nk
+                for (uint32_t e = 0; e < nk; ++e)
+                    if (uni(keys[e]) == pos) return e;
+                return nk;
+            };
+            for (uint32_t j = 0; j < k; ++j) {
+                const uint32_t remaining = n - j;
+                const uint32_t pick = j + (uint32_t)(rng_double(rng) * (double)remaining);
+                if (pick == j) continue;  // swap with itself
+                const uint32_t nk0 = nk;
+                const uint32_t ej = find(j), ep = find(pick);
+                const uint32_t vj = ej < nk0 ? uni(vals[ej]) : j;
+                const uint32_t vp = ep < nk0 ? uni(vals[ep]) : pick;
+                // pool[j] <- pool[pick], pool[pick] <- pool[j]
+                const uint32_t sj = ej < nk0 ? ej : nk++;
+                const uint32_t sp = ep < nk0 ? ep : nk++;
+                if (lead) {
+                    keys[sj] = j;
+                    vals[sj] = vp;
+                    keys[sp] = pick;
+                    vals[sp] = vj;
+                }
+                wave_fence();
+            }
+            for (uint32_t j = 0; j < k; ++j) {  // prefix pool[0..k), then sort it
+                const uint32_t e = find(j);
+                const uint32_t v = e < nk ? uni(vals[e]) : j;
+                if (lead) list[j] = v;
+            }
+            wave_fence();
+            for (uint32_t a = 1; a < k; ++a) {
+                const uint32_t v = uni(list[a]);
+                uint32_t b = a;
+                while (b > 0 && uni(list[b - 1]) > v) {
+                    const uint32_t prev = uni(list[b - 1]);
+                    if (lead) list[b] = prev;
+                    wave_fence();
+                    --b;
+                }
+                if (lead) list[b] = v;
+                wave_fence();
+            }
+            m = k;
+        } else {
+            cptr<double> w = cp(T->w), dp = cp(T->dp);
+            const uint32_t stride = k + 1;
+            uint32_t needed = k;
+            for (uint32_t i = 0; i < n && needed > 0; ++i) {
+                const uint32_t remaining = n - i;
+                double inc;
+                if (needed == remaining) {
+                    inc = 1.0;
+                } else {
+                    const double denom = dp[(size_t)i * stride + needed];
+                    inc = denom > 0.0 ? (w[i] * dp[(size_t)(i + 1) * stride + needed - 1]) / denom : 0.0;
+                }
+                if (rng_double(rng) < inc) {
+                    if (lead) list[m] = i;
+                    ++m;
+                    --needed;
+                }
+            }
+            wave_fence();
+        }
+        // picks are ascending: quantum sites first, then readout entries
+        uint32_t nn = 0;
+        while (nn < m && uni(list[nn]) < n_q) ++nn;
+        for (uint32_t j = nn; j < m; ++j) {
+            const uint32_t v = uni(list[j]);
+            if (lead) list[j] = v - n_q;
+        }
+        wave_fence();
+        kf = 1;
+        kf_list = list;
+        kf_nn = nn;
+        kf_nr = m - nn;
+        kf_rpos = 0;
+        if (nn > 0) {
+            next_noise = uni(list[0]);
+            kf_npos = 1;
+        } else {
+            next_noise = kNoNoise;
+            kf_npos = 0;
+        }
+    }
+
     // ------------------------------------------------------------- dispatch
     // svm_kernels.inl:2183-2646.  Returns when the bytecode ends, a
     // POSTSELECT discards the shot, or a forced outcome is unreachable.
@@ -385,6 +516,7 @@ struct Interp {
             const uint32_t a2 = (uint32_t)(w0 >> 48);
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
+            if (npend && !((kNoAmpOps >> op) & 1ull)) flush_expand();
             switch (op) {
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
                 case CLIFFT_OP_FRAME_CZ: frame_cz(a1, a2); break;
@@ -484,7 +616,16 @@ struct Interp {
                     }
                     const bool has_phase = op != CLIFFT_OP_EXPAND;
                     uint32_t px = f.gx(a1);
-                    sweep_expand<BT>(amp, k, zr, px ? -zi : zi, has_phase ? 1u : 0u, g);
+                    // recorded now, written by flush_expand() together with
+                    // the expansions that follow it
+                    if (npend == kMaxFusedExpand) flush_expand();
+                    if ((threadIdx.x & 63u) == 0) {
+                        pend[2 * npend] = has_phase ? zr : 1.0;
+                        pend[2 * npend + 1] = has_phase ? (px ? -zi : zi) : 0.0;
+                    }
+                    wave_fence();
+                    pend_mask |= (has_phase ? 1u : 0u) << npend;
+                    npend++;
                     k++;
                     scale_magnitude(1.0 / sqrt(2.0));
                     if (has_phase && px) mul_phase(zr, zi);
@@ -552,7 +693,14 @@ struct Interp {
                 }
                 case CLIFFT_OP_READOUT_NOISE: {  // :1937-1959
                     cptr<DevProgram> P = opaque(Pg);
-                    if (rng_double(rng) < cp(P->ro_prob)[i0]) {
+                    bool fire;
+                    if (kf) {
+                        fire = kf_rpos < kf_nr && uni(kf_list[kf_nn + kf_rpos]) == i0;
+                        if (fire) kf_rpos++;
+                    } else {
+                        fire = rng_double(rng) < cp(P->ro_prob)[i0];
+                    }
+                    if (fire) {
                         uint32_t mi = cp(P->ro_meas_idx)[i0];
                         rec_write(meas, mi, rec_read(meas, mi) ^ 1u);
                     }
@@ -587,6 +735,7 @@ struct Interp {
                 default: break;
             }
         }
+        flush_expand();
     }
 };
 

@@ -62,10 +62,14 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
     unsigned char* lds_amp = p;
     if (!GA) p += (size_t)teams_per_block * amp_elems * sizeof(dv2);
     S.red = (lds_f64*)(p);
-    p += 16 * 4 * sizeof(double);
+    S.pend = S.red + 16 * 4 + 16 * wave_in_block;
+    p += kTeamScratchBytes;
     unsigned char* wblock = p + (size_t)wave_in_block * rec_stride;
     unsigned char* rec = wblock;
     typedef __attribute__((address_space(3))) uint8_t lds_u8;
+    // MODE_KFAULT: the last 5 * kMaxForcedFaults words of the wave's block
+    __attribute__((address_space(3))) uint32_t* kf_block =
+        (__attribute__((address_space(3))) uint32_t*)(wblock + rec_stride - 20u * kMaxForcedFaults);
     {
         cptr<DevProgram> P = opaque(Pg);
         S.meas = (lds_u8*)(rec);
@@ -86,6 +90,9 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
         for (uint32_t i = wl; i < rec_stride / 4; i += 64) reinterpret_cast<uint32_t*>(wblock)[i] = 0u;
         wave_fence();
         S.f.init(wblock);
+        S.kf = 0;
+        S.npend = 0;
+        S.pend_mask = 0;
         S.dust = 0;
         S.discarded = 0;
         S.reachable = 1;
@@ -116,8 +123,12 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
             S.k = 0;
             S.rng = rng_for_shot(seed, shot);
             if (S.g.lane == 0) S.amp[0] = mk(1.0, 0.0);
-            S.next_noise = 0;
-            S.draw_next_noise(P);
+            if (mode == MODE_KFAULT) {  // svm.cc:602-606: no gap draw, fault set first
+                S.kfault_prepare(cp(A->kf), kf_block);
+            } else {
+                S.next_noise = 0;
+                S.draw_next_noise(P);
+            }
             if (mode == MODE_FORCED) S.forced = cp(A->forced_records) + s * P->num_meas;
         }
         team_sync<BT>();

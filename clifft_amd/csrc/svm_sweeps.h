@@ -285,6 +285,33 @@ CSVM_SWEEP void sweep_expand(AmpT* amp, uint32_t k, double pr, double pi, uint32
     team_sync<BT>();
 }
 
+// Several consecutive EXPANDs in one pass (the opcodes between them touch
+// only the frame, gamma, the PRNG or records).  Expansion b (0-based,
+// phase z_b when bit b of `mask` is set, else a plain copy) maps a state of
+// 2^(k0+b) entries to twice that, so
+//     amp[i + t 2^k0] = (((amp[i] z_b1) z_b2) ... z_bj)   for bits b1 < ... < bj of t,
+// the very products the one-at-a-time sweeps form: one read and 2^m - 1
+// writes per source entry instead of 2^(m+1) - 2 of each.
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_expand_multi(AmpT* amp, uint32_t k0, uint32_t m, const lds_f64* ph, uint32_t mask, TeamGeom g) {
+    k0 = uni(k0); m = uni(m); mask = uni(mask); g = uni_geom(g);
+    dv2 z[kMaxFusedExpand];
+#pragma unroll
+    for (uint32_t b = 0; b < kMaxFusedExpand; ++b) z[b] = b < m ? mk(unid(ph[2 * b]), unid(ph[2 * b + 1])) : mk(1.0, 0.0);
+    const uint32_t K = 1u << k0, T = 1u << m;
+    for (uint32_t i = g.lane; i < K; i += g.size) {
+        const dv2 a = amp[i];
+        for (uint32_t t = 1; t < T; ++t) {
+            dv2 v = a;
+#pragma unroll
+            for (uint32_t b = 0; b < kMaxFusedExpand; ++b)
+                if (((t >> b) & 1u) && ((mask >> b) & 1u)) v = cmul(v, z[b]);
+            amp[i + t * K] = v;
+        }
+    }
+    team_sync<BT>();
+}
+
 template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_scale(AmpT* amp, uint32_t k, double s, TeamGeom g) {
     k = uni(k); s = unid(s); g = uni_geom(g);

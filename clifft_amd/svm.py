@@ -8,6 +8,8 @@
         (src/clifft/api/record_probabilities.cc:96-191)
     State / execute(program, state)         -> single shot from a preset state
         (src/python/bindings.cc:1040-1080, svm.cc:158-164)
+    sample_k / sample_k_survivors / noise_site_probabilities
+        -> exactly-k-fault importance sampling (svm.h:366-378, svm.cc:347-711)
 
 All execution goes through libclifft_svm.so; nothing here falls back to the
 CPU.  Seeding: shot s draws from its own xoshiro256++ stream (SplitMix64
@@ -243,3 +245,62 @@ def execute(program, state: State) -> None:
     state.forced_reachable = bool(s.forced_reachable)
     state.forced_log_probability = s.forced_log_probability
     state.dust_clamps = s.dust_clamps
+
+
+def _check_k(rc: int, what: str) -> None:
+    if rc == _lib.E_INVALID:
+        raise ValueError(_lib.lib().clifft_svm_last_error().decode(errors="replace"))
+    _lib.check(rc, what)
+
+
+def sample_k(program, shots: int, k: int, seed=None, packed: bool = False) -> SampleResult:
+    """Every shot has exactly k faults (noise sites + readout entries); raises
+    ValueError for an infeasible k, like the reference (svm.cc:525-546)."""
+    dp = device_program(program)
+    prog = dp.program
+    shots = int(shots)
+
+    def cols(n):
+        return (n + 7) // 8 if packed else n
+
+    m = np.zeros((shots, cols(prog.num_measurements)), np.uint8)
+    d = np.zeros((shots, cols(prog.num_detectors)), np.uint8)
+    o = np.zeros((shots, cols(prog.num_observables)), np.uint8)
+    e = np.zeros((shots, prog.num_exp_vals), np.float64)
+    rec = _lib.Records(_ptr(m), _ptr(d), _ptr(o), _ptr(e), None)
+    flags = _lib.REC_PACKED if packed else 0
+    _check_k(_lib.lib().clifft_svm_sample_k(dp.handle, shots, int(k), _seed(seed), flags, ctypes.byref(rec), None),
+             "sample_k")
+    return SampleResult(m, d, o, exp_vals=e)
+
+
+def sample_k_survivors(program, shots: int, k: int, seed=None, keep_records: bool = False) -> SampleResult:
+    dp = device_program(program)
+    prog = dp.program
+    shots = int(shots)
+    ones = np.zeros(max(prog.num_observables, 1), np.uint64)
+    summ = _lib.Survivors(shots, 0, 0, ones.ctypes.data)
+    m = np.zeros((shots, prog.num_measurements), np.uint8)
+    d = np.zeros((shots, prog.num_detectors), np.uint8)
+    o = np.zeros((shots, prog.num_observables), np.uint8)
+    e = np.zeros((shots, prog.num_exp_vals), np.float64)
+    rec = _lib.Records(_ptr(m), _ptr(d), _ptr(o), _ptr(e), None)
+    _check_k(_lib.lib().clifft_svm_sample_k_survivors(dp.handle, shots, int(k), _seed(seed), 0, ctypes.byref(summ),
+                                                      ctypes.byref(rec) if keep_records else None, None),
+             "sample_k_survivors")
+    n = int(summ.passed_shots)
+    if keep_records:
+        m, d, o, e = m[:n], d[:n], o[:n], e[:n]
+    else:
+        m, d, o, e = m[:0], d[:0], o[:0], e[:0]
+    return SampleResult(m, d, o, total_shots=shots, passed_shots=n, logical_errors=int(summ.logical_errors),
+                        observable_ones=ones[:prog.num_observables].copy(), exp_vals=e)
+
+
+def noise_site_probabilities(program) -> np.ndarray:
+    dp = device_program(program)
+    n = ctypes.c_uint32(0)
+    _lib.check(_lib.lib().clifft_svm_noise_site_probabilities(dp.handle, None, ctypes.byref(n)), "noise_sites")
+    out = np.zeros(n.value, np.float64)
+    _lib.check(_lib.lib().clifft_svm_noise_site_probabilities(dp.handle, _ptr(out), ctypes.byref(n)), "noise_sites")
+    return out

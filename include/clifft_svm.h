@@ -316,6 +316,26 @@ int clifft_svm_sample_survivors(clifft_svm_program* prog, uint64_t shots, uint64
                                 uint32_t flags, clifft_svm_survivors* summary,
                                 const clifft_svm_records* out, void* stream);
 
+/* Importance sampling: every shot has exactly k faults, the fault set drawn
+ * from the conditional Poisson-binomial law over the quantum noise sites and
+ * readout entries (or uniformly when all their probabilities are equal).
+ * Replaces clifft::sample_k / sample_k_survivors (src/clifft/svm/svm.h:366-373,
+ * svm.cc:347-711).  Shot s uses the per-shot stream of (seed, s) as in
+ * clifft_svm_sample; shot 0 equals the reference's first shot (the uniform
+ * sampler restarts its pool from the identity every shot, which the
+ * reference does only for its first shot).  Infeasible k returns
+ * CLIFFT_E_INVALID with the reference's message; k above 64 is
+ * CLIFFT_E_UNSUPPORTED. */
+int clifft_svm_sample_k(clifft_svm_program* prog, uint64_t shots, uint32_t k, uint64_t seed,
+                        uint32_t flags, const clifft_svm_records* out, void* stream);
+int clifft_svm_sample_k_survivors(clifft_svm_program* prog, uint64_t shots, uint32_t k,
+                                  uint64_t seed, uint32_t flags, clifft_svm_survivors* summary,
+                                  const clifft_svm_records* out, void* stream);
+/* Per-site fault probabilities: quantum sites (sum of channel probabilities)
+ * then readout entries (clifft::noise_site_probabilities, svm.h:378,
+ * svm.cc:356-369).  `out` may be NULL to query *count. */
+int clifft_svm_noise_site_probabilities(const clifft_svm_program* prog, double* out, uint32_t* count);
+
 /* Exact log-probabilities of measurement records (forced execution).
  * Replaces clifft::record_probabilities (src/clifft/svm/svm.h:405,
  * src/clifft/api/record_probabilities.cc:96-191): measurement opcodes are

@@ -63,6 +63,8 @@ def lib(fast: bool = False):
             "orc_record_log_probs": (ctypes.c_int, [vp, vp, u64, vp]),
             "orc_execute_forced_faults": (ctypes.c_int, [vp, vp, vp, u32, vp, u32]),
             "orc_max_threads": (ctypes.c_int, []),
+            "orc_sample_k": (ctypes.c_int, [vp, u64, u32, u64, ctypes.c_int, u64, vp, vp, vp, vp, vp]),
+            "orc_noise_site_probabilities": (u32, [vp, vp]),
         }
         for fn, (res, args) in sigs.items():
             f = getattr(L, fn)
@@ -232,6 +234,36 @@ def sample(prog, shots, seed=0, per_shot=True, first_shot=0, fast=False):
                       _p(m), _p(d), _p(o), _p(e), _p(disc))
     assert rc == 0
     return m, d, o, e, disc
+
+
+def sample_k(prog, shots, k, seed=0, per_shot=True, first_shot=0):
+    """Oracle sample_k (src/clifft/svm/svm.cc:549-622): exactly k forced
+    faults per shot.  Raises ValueError where the reference raises
+    std::invalid_argument."""
+    L = lib()
+    v = prog.view()
+    m = np.zeros((shots, prog.num_measurements), np.uint8)
+    d = np.zeros((shots, prog.num_detectors), np.uint8)
+    o = np.zeros((shots, prog.num_observables), np.uint8)
+    e = np.zeros((shots, prog.num_exp_vals), np.float64)
+    disc = np.zeros(shots, np.uint8)
+    rc = L.orc_sample_k(ctypes.byref(v), shots, k, seed & (2**64 - 1), 1 if per_shot else 0, first_shot,
+                        _p(m), _p(d), _p(o), _p(e), _p(disc))
+    if rc == -2:
+        raise ValueError(f"k ({k}) exceeds total fault sites")
+    if rc == -3:
+        raise ValueError(f"k-fault stratum k={k} has zero probability mass")
+    assert rc == 0
+    return m, d, o, e, disc
+
+
+def noise_site_probabilities(prog):
+    L = lib()
+    v = prog.view()
+    n = L.orc_noise_site_probabilities(ctypes.byref(v), None)
+    out = np.zeros(n, np.float64)
+    L.orc_noise_site_probabilities(ctypes.byref(v), _p(out))
+    return out
 
 
 def sample_parallel(prog, shots, seed=0, threads=0, fast=True):

@@ -1091,6 +1091,189 @@ ORC_EXPORT int orc_execute_forced_faults(const prog_t* p, orc_state* st, const u
     return 0;
 }
 
+/* ---------------------------------------------------------------------
+ * Importance sampling with exactly k forced faults per shot
+ * (svm.cc:347-622).  Sites are the quantum noise sites followed by the
+ * readout entries.
+ * ------------------------------------------------------------------- */
+
+/* svm.cc:356-369 */
+static double* site_probabilities(const prog_t* p, uint32_t* n_out) {
+    uint32_t n = p->num_sites + p->num_readout;
+    double* probs = (double*)malloc((n + 1) * sizeof(double));
+    for (uint32_t s = 0; s < p->num_sites; ++s) {
+        double acc = 0.0;
+        for (uint32_t c = p->site_chan_off[s]; c < p->site_chan_off[s + 1]; ++c) acc += p->chan_prob[c];
+        probs[s] = acc;
+    }
+    for (uint32_t e = 0; e < p->num_readout; ++e) probs[p->num_sites + e] = p->ro_prob[e];
+    *n_out = n;
+    return probs;
+}
+
+ORC_EXPORT uint32_t orc_noise_site_probabilities(const prog_t* p, double* out) {
+    uint32_t n;
+    double* probs = site_probabilities(p, &n);
+    if (out) memcpy(out, probs, n * sizeof(double));
+    free(probs);
+    return n;
+}
+
+/* svm.cc:377-392: odds ratios w = p/(1-p), p clamped to [0, 1-1e-15],
+ * rescaled to mean 1 */
+static void odds_ratios(const double* probs, uint32_t n, double* w) {
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double q = probs[i] < 0.0 ? 0.0 : (probs[i] > 1.0 - 1e-15 ? 1.0 - 1e-15 : probs[i]);
+        w[i] = q / (1.0 - q);
+        sum += w[i];
+    }
+    if (sum > 0.0) {
+        double scale = (double)n / sum;
+        for (uint32_t i = 0; i < n; ++i) w[i] *= scale;
+    }
+}
+
+/* svm.cc:411-431: dp[i*(k+1)+j] = e_j(w[i..n)) */
+static void dp_table(const double* w, uint32_t n, uint32_t k, double* dp) {
+    const uint32_t stride = k + 1;
+    memset(dp, 0, (size_t)(n + 1) * stride * sizeof(double));
+    for (uint32_t i = 0; i <= n; ++i) dp[(size_t)i * stride] = 1.0;
+    for (int i = (int)n - 1; i >= 0; --i) {
+        uint32_t remaining = n - (uint32_t)i, max_j = remaining < k ? remaining : k;
+        for (uint32_t j = 1; j <= max_j; ++j)
+            dp[(size_t)i * stride + j] = dp[(size_t)(i + 1) * stride + j] + w[i] * dp[(size_t)(i + 1) * stride + j - 1];
+    }
+}
+
+static int all_equal(const double* probs, uint32_t n) {
+    for (uint32_t i = 1; i < n; ++i)
+        if (probs[i] != probs[0]) return 0;
+    return 1;
+}
+
+/* svm.cc:525-546; returns 0 or -2 (k above the site count) / -3 (zero mass) */
+static int stratum_ok(const double* probs, uint32_t n, uint32_t k) {
+    if (k > n) return -2;
+    uint32_t certain = 0, impossible = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (probs[i] <= 0.0)
+            impossible++;
+        else if (probs[i] >= 1.0)
+            certain++;
+    }
+    if (k < certain || k > n - impossible) return -3;
+    return 0;
+}
+
+static int cmp_u32(const void* a, const void* b) {
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : (x > y);
+}
+
+/* svm.cc:435-520: choose the fault set of one shot (DP walk, or partial
+ * Fisher-Yates on `pool` with the chosen prefix sorted) and arm the state */
+static void prepare_k_shot(orc_state* st, const prog_t* p, uint32_t k, uint32_t n, int uniform, const double* w,
+                           const double* dp, uint32_t* pool, uint32_t* picks) {
+    uint32_t m = 0;
+    if (uniform) {
+        for (uint32_t j = 0; j < k; ++j) {
+            uint32_t remaining = n - j;
+            uint32_t pick = j + (uint32_t)(rng_double(&st->rng) * remaining);
+            uint32_t t = pool[j];
+            pool[j] = pool[pick];
+            pool[pick] = t;
+        }
+        qsort(pool, k, sizeof(uint32_t), cmp_u32);
+        for (uint32_t j = 0; j < k; ++j) picks[m++] = pool[j];
+    } else {
+        const uint32_t stride = k + 1;
+        uint32_t needed = k;
+        for (uint32_t i = 0; i < n && needed > 0; ++i) {
+            double inc;
+            uint32_t remaining = n - i;
+            if (needed == remaining) {
+                inc = 1.0;
+            } else {
+                double denom = dp[(size_t)i * stride + needed];
+                inc = denom > 0.0 ? (w[i] * dp[(size_t)(i + 1) * stride + needed - 1]) / denom : 0.0;
+            }
+            if (rng_double(&st->rng) < inc) {
+                picks[m++] = i;
+                needed--;
+            }
+        }
+    }
+    uint32_t nn = 0, nr = 0;
+    free(st->ff_noise);
+    free(st->ff_ro);
+    st->ff_noise = (uint32_t*)malloc((m + 1) * 4);
+    st->ff_ro = (uint32_t*)malloc((m + 1) * 4);
+    for (uint32_t j = 0; j < m; ++j) {
+        if (picks[j] < p->num_sites)
+            st->ff_noise[nn++] = picks[j];
+        else
+            st->ff_ro[nr++] = picks[j] - p->num_sites;
+    }
+    st->ff_n_noise = nn;
+    st->ff_n_ro = nr;
+    st->ff_ro_pos = 0;
+    st->ff_active = 1;
+    if (nn > 0) {
+        st->next_noise_idx = st->ff_noise[0];
+        st->ff_noise_pos = 1;
+    } else {
+        st->next_noise_idx = 0xFFFFFFFFu;
+        st->ff_noise_pos = 0;
+    }
+}
+
+/* svm.cc:549-622 (sample_k).  mode 0: one stream and a Fisher-Yates pool
+ * carried across shots (the reference); mode 1: per-shot streams and a
+ * fresh identity pool per shot (the device convention; shot 0 agrees with
+ * mode 0).  Returns 0, -2 (k exceeds the number of sites) or -3 (the
+ * k-fault stratum has zero probability mass). */
+ORC_EXPORT int orc_sample_k(const prog_t* p, uint64_t shots, uint32_t k, uint64_t seed, int mode, uint64_t first_shot,
+                            uint8_t* meas, uint8_t* det, uint8_t* obs, double* ev, uint8_t* discarded) {
+    uint32_t n;
+    double* probs = site_probabilities(p, &n);
+    int rc = stratum_ok(probs, n, k);
+    if (rc) {
+        free(probs);
+        return rc;
+    }
+    const int uniform = all_equal(probs, n);
+    double* w = (double*)calloc(n + 1, sizeof(double));
+    double* dp = (double*)calloc((size_t)(n + 1) * (k + 1), sizeof(double));
+    uint32_t* pool = (uint32_t*)malloc((n + 1) * 4);
+    uint32_t* picks = (uint32_t*)malloc((k + 1) * 4);
+    if (!uniform) {
+        odds_ratios(probs, n, w);
+        dp_table(w, n, k, dp);
+    }
+    for (uint32_t i = 0; i < n; ++i) pool[i] = i;
+    orc_state* st = state_for(p, seed);
+    for (uint64_t s = 0; s < shots; ++s) {
+        if (s > 0) orc_state_reset(st);
+        if (mode == 1) {
+            rng_seed_shot(&st->rng, seed, first_shot + s);
+            for (uint32_t i = 0; i < n; ++i) pool[i] = i;
+        }
+        prepare_k_shot(st, p, k, n, uniform, w, dp, pool, picks);
+        orc_execute(p, st);
+        st->ff_active = 0;
+        copy_out(p, st, s, meas, det, obs, ev);
+        if (discarded) discarded[s] = (uint8_t)st->discarded;
+    }
+    orc_state_free(st);
+    free(probs);
+    free(w);
+    free(dp);
+    free(pool);
+    free(picks);
+    return 0;
+}
+
 /* Seed the state's rng with the device per-shot convention. */
 ORC_EXPORT void orc_state_seed_shot(orc_state* st, uint64_t seed, uint64_t shot) {
     rng_seed_shot(&st->rng, seed, shot);
