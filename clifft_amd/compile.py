@@ -14,7 +14,7 @@ import subprocess
 
 import numpy as np
 
-from .program import INSTR_DTYPE, U2_DTYPE, Program, ProgramView
+from .program import INSTR_DTYPE, U2_DTYPE, U4_DTYPE, Program, ProgramView
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libclifft_compile.so")
@@ -106,6 +106,7 @@ def compile(text: str, *, fusion: bool = True, keep_rotations: bool = False, pas
             obs_off=_arr(v.obs_off, v.num_obs_lists + 1, np.uint32),
             obs_tgt=_arr(v.obs_tgt, int(_arr(v.obs_off, v.num_obs_lists + 1, np.uint32)[-1]), np.uint32),
             u2_nodes=_arr(v.u2_nodes, v.num_u2, U2_DTYPE),
+            u4_nodes=_arr(v.u4_nodes, v.num_u4, U4_DTYPE),
         )
         npar = L.clifft_compiled_num_params(h)
         prog.param_instr = _arr(ctypes.cast(L.clifft_compiled_param_instr(h), ctypes.c_void_p).value, npar, np.uint32)

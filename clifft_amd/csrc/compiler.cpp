@@ -734,6 +734,198 @@ void pass_u2(Code& bc, std::vector<int32_t>& tag, std::vector<clifft_u2_node>& n
     tag.swap(ot);
 }
 
+// Two-axis (tile) fusion into U4 nodes (behaviour of
+// src/clifft/optimizer/tile_axis_fusion_pass.cc:393-458, node semantics
+// :208-391).  A run starts at a two-axis array op on axes {lo, hi} and
+// extends while each instruction acts on lo and/or hi only (single-axis
+// array or frame ops on either axis, two-axis ops on exactly {lo, hi}); a
+// run holding >= 3 array ops becomes one ARRAY_U4.  Its node stores, for
+// each of the 16 incoming frame states s = (pz_hi, px_hi, pz_lo, px_lo), the
+// 4x4 product acting on the tile index (b_hi << 1) | b_lo, the gamma factor
+// and the outgoing frame state.  Parameterised rotations stay unfused when
+// `keep_rot` is set.
+struct TileFrame {
+    int px[2], pz[2];  // [0] = lo axis, [1] = hi axis
+};
+
+int tile_kind(const clifft_instruction& in, bool keep_rot) {  // 1: one axis, 2: two axes, 0: neither
+    switch (in.opcode) {
+        case CLIFFT_OP_ARRAY_H: case CLIFFT_OP_ARRAY_S: case CLIFFT_OP_ARRAY_S_DAG: case CLIFFT_OP_ARRAY_T:
+        case CLIFFT_OP_ARRAY_T_DAG: case CLIFFT_OP_FRAME_H: case CLIFFT_OP_FRAME_S: case CLIFFT_OP_FRAME_S_DAG:
+            return 1;
+        case CLIFFT_OP_ARRAY_ROT:
+            return keep_rot ? 0 : 1;
+        case CLIFFT_OP_ARRAY_CNOT: case CLIFFT_OP_ARRAY_CZ: case CLIFFT_OP_ARRAY_SWAP: case CLIFFT_OP_FRAME_CNOT:
+        case CLIFFT_OP_FRAME_CZ: case CLIFFT_OP_FRAME_SWAP:
+            return 2;
+        default:
+            return 0;
+    }
+}
+
+bool tile_array_op(uint8_t op) {
+    return (op >= CLIFFT_OP_ARRAY_CNOT && op <= CLIFFT_OP_ARRAY_SWAP) ||
+           (op >= CLIFFT_OP_ARRAY_H && op <= CLIFFT_OP_ARRAY_ROT);
+}
+
+// Left-multiply the 4x4 M (row-major) by a 2x2 u acting on tile bit `pos`.
+void tile_apply_1q(cd M[16], const cd u[4], int pos) {
+    const int bit = 1 << pos;
+    for (int r0 = 0; r0 < 4; ++r0) {
+        if (r0 & bit) continue;
+        const int r1 = r0 | bit;
+        for (int col = 0; col < 4; ++col) {
+            cd a = M[r0 * 4 + col], b = M[r1 * 4 + col];
+            M[r0 * 4 + col] = u[0] * a + u[1] * b;
+            M[r1 * 4 + col] = u[2] * a + u[3] * b;
+        }
+    }
+}
+
+void tile_swap_rows(cd M[16], int r0, int r1) {
+    for (int col = 0; col < 4; ++col) std::swap(M[r0 * 4 + col], M[r1 * 4 + col]);
+}
+
+void simulate_tile(const clifft_instruction& in, uint16_t lo, TileFrame& f, cd M[16], cd& g) {
+    const cd I(0, 1), w(kInvSqrt2, kInvSqrt2), wc(kInvSqrt2, -kInvSqrt2);
+    const int pos = in.axis_1 == lo ? 0 : 1;  // single-axis ops: which tile bit
+    int& px = f.px[pos];
+    int& pz = f.pz[pos];
+    const bool array = tile_array_op(in.opcode);
+    switch (in.opcode) {
+        case CLIFFT_OP_ARRAY_H:
+        case CLIFFT_OP_FRAME_H: {
+            if (px && pz) g *= -1.0;
+            std::swap(px, pz);
+            if (array) {
+                const cd u[4] = {kInvSqrt2, kInvSqrt2, kInvSqrt2, -kInvSqrt2};
+                tile_apply_1q(M, u, pos);
+            }
+            return;
+        }
+        case CLIFFT_OP_ARRAY_S:
+        case CLIFFT_OP_ARRAY_S_DAG:
+        case CLIFFT_OP_FRAME_S:
+        case CLIFFT_OP_FRAME_S_DAG: {
+            const bool dag = in.opcode == CLIFFT_OP_ARRAY_S_DAG || in.opcode == CLIFFT_OP_FRAME_S_DAG;
+            if (px) g *= dag ? -I : I;
+            pz ^= px;
+            if (array) {
+                const cd u[4] = {1.0, 0.0, 0.0, dag ? -I : I};
+                tile_apply_1q(M, u, pos);
+            }
+            return;
+        }
+        case CLIFFT_OP_ARRAY_T:
+        case CLIFFT_OP_ARRAY_T_DAG:
+        case CLIFFT_OP_ARRAY_ROT: {
+            cd z = in.opcode == CLIFFT_OP_ARRAY_T ? w : (in.opcode == CLIFFT_OP_ARRAY_T_DAG ? wc : cd(in.p.math.weight_re, in.p.math.weight_im));
+            const cd u[4] = {1.0, 0.0, 0.0, px ? std::conj(z) : z};
+            if (px) g *= z;
+            tile_apply_1q(M, u, pos);
+            return;
+        }
+        default:
+            break;
+    }
+    // two-axis ops on exactly {lo, hi}
+    const int cpos = in.axis_1 == lo ? 0 : 1, tpos = 1 - cpos;
+    switch (in.opcode) {
+        case CLIFFT_OP_ARRAY_CNOT:
+        case CLIFFT_OP_FRAME_CNOT:
+            f.px[tpos] ^= f.px[cpos];
+            f.pz[cpos] ^= f.pz[tpos];
+            if (in.opcode == CLIFFT_OP_ARRAY_CNOT) {
+                // rows with the control bit set swap their target bit
+                const int cb = 1 << cpos, tb = 1 << tpos;
+                tile_swap_rows(M, cb, cb | tb);
+            }
+            return;
+        case CLIFFT_OP_ARRAY_CZ:
+        case CLIFFT_OP_FRAME_CZ:
+            if (f.px[0] && f.px[1]) g *= -1.0;
+            f.pz[1] ^= f.px[0];
+            f.pz[0] ^= f.px[1];
+            if (in.opcode == CLIFFT_OP_ARRAY_CZ)
+                for (int col = 0; col < 4; ++col) M[12 + col] = -M[12 + col];
+            return;
+        case CLIFFT_OP_ARRAY_SWAP:
+        case CLIFFT_OP_FRAME_SWAP:
+            std::swap(f.px[0], f.px[1]);
+            std::swap(f.pz[0], f.pz[1]);
+            if (in.opcode == CLIFFT_OP_ARRAY_SWAP) tile_swap_rows(M, 1, 2);
+            return;
+        default:
+            return;
+    }
+}
+
+void pass_u4(Code& bc, std::vector<int32_t>& tag, std::vector<clifft_u4_node>& nodes, bool keep_rot) {
+    Code out;
+    std::vector<int32_t> ot;
+    for (size_t i = 0; i < bc.size();) {
+        const clifft_instruction& head = bc[i];
+        const bool starts = head.opcode == CLIFFT_OP_ARRAY_CNOT || head.opcode == CLIFFT_OP_ARRAY_CZ ||
+                            head.opcode == CLIFFT_OP_ARRAY_SWAP;
+        if (!starts) {
+            out.push_back(head);
+            ot.push_back(tag[i]);
+            ++i;
+            continue;
+        }
+        const uint16_t lo = std::min(head.axis_1, head.axis_2), hi = std::max(head.axis_1, head.axis_2);
+        size_t j = i + 1;
+        int n_array = 1;
+        while (j < bc.size()) {
+            const clifft_instruction& in = bc[j];
+            const int kind = tile_kind(in, keep_rot);
+            bool fits = false;
+            if (kind == 1) fits = in.axis_1 == lo || in.axis_1 == hi;
+            if (kind == 2) fits = std::min(in.axis_1, in.axis_2) == lo && std::max(in.axis_1, in.axis_2) == hi;
+            if (!fits) break;
+            if (tile_array_op(in.opcode)) ++n_array;
+            ++j;
+        }
+        if (n_array >= 3) {
+            clifft_u4_node node;
+            std::memset(&node, 0, sizeof(node));
+            for (int s = 0; s < 16; ++s) {
+                TileFrame f{{s & 1, (s >> 2) & 1}, {(s >> 1) & 1, (s >> 3) & 1}};
+                cd M[16] = {};
+                for (int d = 0; d < 4; ++d) M[d * 5] = 1.0;
+                cd g(1.0, 0.0);
+                for (size_t r = i; r < j; ++r) simulate_tile(bc[r], lo, f, M, g);
+                clifft_u4_entry& e = node.entries[s];
+                for (int r = 0; r < 4; ++r)
+                    for (int col = 0; col < 4; ++col) {
+                        e.matrix[r][col][0] = M[r * 4 + col].real();
+                        e.matrix[r][col][1] = M[r * 4 + col].imag();
+                    }
+                e.gamma[0] = g.real();
+                e.gamma[1] = g.imag();
+                e.out_state = (uint8_t)((f.pz[1] << 3) | (f.px[1] << 2) | (f.pz[0] << 1) | f.px[0]);
+            }
+            clifft_instruction in;
+            std::memset(&in, 0, sizeof(in));
+            in.opcode = CLIFFT_OP_ARRAY_U4;
+            in.axis_1 = lo;
+            in.axis_2 = hi;
+            in.p.u.cp_idx = (uint32_t)nodes.size();
+            nodes.push_back(node);
+            out.push_back(in);
+            ot.push_back(-1);
+        } else {
+            for (size_t r = i; r < j; ++r) {
+                out.push_back(bc[r]);
+                ot.push_back(tag[r]);
+            }
+        }
+        i = j;
+    }
+    bc.swap(out);
+    tag.swap(ot);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -748,6 +940,7 @@ struct clifft_compiled {
     std::vector<uint32_t> site_off, det_off, det_tgt, obs_off, obs_tgt;
     std::vector<double> chan_prob, hazards;
     std::vector<clifft_u2_node> u2;
+    std::vector<clifft_u4_node> u4;
     std::vector<uint32_t> param_instr;
     std::vector<int8_t> param_sign;
     cd global_weight{1.0, 0.0};
@@ -926,8 +1119,11 @@ clifft_compiled* compile_impl(const char* text, uint32_t options) {
         pass_multi_gate(bc, tag);
         pass_expand_phase(bc, tag);
         pass_swap_meas(bc, tag);
-        if (!(options & CLIFFT_COMPILE_NO_FUSION))
-            pass_u2(bc, tag, c->u2, (options & CLIFFT_COMPILE_KEEP_ROTATIONS) != 0);
+        if (!(options & CLIFFT_COMPILE_NO_FUSION)) {
+            const bool keep_rot = (options & CLIFFT_COMPILE_KEEP_ROTATIONS) != 0;
+            pass_u4(bc, tag, c->u4, keep_rot);
+            pass_u2(bc, tag, c->u2, keep_rot);
+        }
     }
     c->num_qubits = n;
     c->words = L.W;
@@ -1020,6 +1216,8 @@ int clifft_compiled_view(const clifft_compiled* c, clifft_program_view* v) {
     v->num_obs_lists = (uint32_t)c->obs_off.size() - 1;
     v->u2_nodes = c->u2.data();
     v->num_u2 = (uint32_t)c->u2.size();
+    v->u4_nodes = c->u4.data();
+    v->num_u4 = (uint32_t)c->u4.size();
     return CLIFFT_OK;
 }
 

@@ -124,3 +124,47 @@ def test_blob_round_trip():
     assert back.bytecode.tobytes() == prog.bytecode.tobytes()
     assert back.u2_nodes.tobytes() == prog.u2_nodes.tobytes()
     assert (back.peak_rank, back.num_measurements) == (prog.peak_rank, prog.num_measurements)
+
+
+def tile_circuit(rng, noisy):
+    """Two active qubits hammered by a long run of one- and two-qubit gates
+    on that pair (what the tile-axis fusion pass fuses), plus a spectator.
+    With `noisy`, depolarizing noise right before the run makes the frame
+    entering the fused node random (all 16 frame states occur)."""
+    lines = ["H 0 1 2", "T 0 1"]
+    if noisy:
+        lines += ["DEPOLARIZE2(0.3) 0 1", "DEPOLARIZE1(0.3) 0 1"]
+    for _ in range(8):
+        lines.append(str(rng.choice(["CX 0 1", "CX 1 0", "CZ 0 1", "SWAP 0 1"])))
+        lines.append(str(rng.choice(["T 0", "T 1", "T_DAG 0", "T_DAG 1"])))
+        lines.append(str(rng.choice(["H 0", "H 1", "S 0", "S_DAG 1", "Z 0"])))
+    lines.append("CX 2 0")
+    return lines
+
+
+def _tile_text(seed, noisy, tail):
+    """First circuit of the seeded family whose compiled form has a U4 node."""
+    from clifft_amd.compile import compile as cc
+    from clifft_amd.program import Op
+    rng = np.random.default_rng(seed)
+    for _ in range(50):
+        text = "\n".join(tile_circuit(rng, noisy) + [tail])
+        if (cc(text).bytecode["opcode"] == Op.ARRAY_U4).sum() >= 1:
+            return text
+    raise AssertionError("no circuit with a tile run")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tile_fusion_matches_dense_and_unfused(orc, seed):
+    from clifft_amd.compile import compile as cc
+    from dense_circuit import run
+    text = _tile_text(seed, False, "EXP_VAL X0*X1 Z0 Y1 Z0*Z1 X0*Y1*Z2")
+    _, _, _, e, _ = orc.sample(cc(text), 1, seed=0)
+    np.testing.assert_allclose(e[0], run(text)[1], atol=1e-12, rtol=0)
+    # noisy variant: every incoming frame state of the node is exercised;
+    # the fused and unfused programs draw the same random numbers
+    text = _tile_text(seed, True, "M 0 1 2")
+    f, u = cc(text), cc(text, fusion=False)
+    a = orc.sample(f, 3000, seed=seed)
+    b = orc.sample(u, 3000, seed=seed)
+    assert np.array_equal(a[0], b[0])
