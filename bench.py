@@ -58,6 +58,21 @@ def build_program(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def measured_peaks():
+    """LDS (read + write) and HBM copy bandwidth measured on this device with
+    the probes in clifft_amd/libclifft_probe.so (GB/s), or {} if unbuilt."""
+    import ctypes
+    path = os.path.join(ROOT, "clifft_amd", "libclifft_probe.so")
+    if not os.path.exists(path):
+        return {}
+    L = ctypes.CDLL(path)
+    L.clifft_probe_lds.restype = ctypes.c_double
+    L.clifft_probe_lds.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.clifft_probe_copy.restype = ctypes.c_double
+    L.clifft_probe_copy.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    return {"lds_probe_GBps": L.clifft_probe_lds(256 * 8, 200), "hbm_copy_probe_GBps": L.clifft_probe_copy(4 << 30, 5)}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -387,12 +402,13 @@ def main():
         "config": {"workload": args.workload, "description": desc, "shots_per_step": shots,
                    "num_qubits": prog.num_qubits, "peak_rank": prog.peak_rank,
                    "instructions": int(len(prog.bytecode)), "array_ops": int(info["num_array_ops"]),
-                   "tier": ["lds", "global"][info["tier"]], "team_waves": info["team_waves"],
+                   "tier": ["team-lds", "team-global", "lane"][info["tier"]], "team_waves": info["team_waves"],
                    "shots_per_block": info["shots_per_block"], "records": "bit-packed, device memory",
                    "parallelism": "shots"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "measured_peaks": measured_peaks()},
         "backend": _lib.backend(),
     }
     if not args.no_cpu:

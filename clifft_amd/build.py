@@ -20,6 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 LIB_SVM = os.path.join(HERE, "libclifft_svm.so")
 SVM_SOURCES = ["svm_kernels.hip", "svm_host.cpp"]
+LIB_PROBE = os.path.join(HERE, "libclifft_probe.so")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -47,8 +48,17 @@ def build_svm(verbose: bool = True) -> str:
     return LIB_SVM
 
 
+def build_probe() -> str:
+    """Bandwidth probes for bench.py (not part of the executor)."""
+    tmp = LIB_PROBE + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "probe.hip")])
+    os.replace(tmp, LIB_PROBE)
+    return LIB_PROBE
+
+
 def build_all() -> None:
     build_svm()
+    build_probe()
     from clifft_amd import compile as _compile
     _compile.build()
 

@@ -1,0 +1,100 @@
+// probe.hip -- bandwidth probes used only by bench.py to put measured peaks
+// next to the published ones: LDS read+write bandwidth of the whole chip and
+// an HBM stream copy.  Built into libclifft_probe.so, separate from the
+// executor.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+// Each lane streams 16-byte elements through its own conflict-free LDS
+// columns (lane-consecutive addresses), one read and one write per element.
+__global__ void __launch_bounds__(256) lds_probe_kernel(double* out, int iters) {
+    __shared__ dv2 buf[256 * 16];  // 64 KiB: two workgroups per CU
+    const uint32_t t = threadIdx.x;
+    for (int j = 0; j < 16; ++j) {
+        dv2 v;
+        v.x = (double)(t + j);
+        v.y = 1.0;
+        buf[t + 256 * j] = v;
+    }
+    __syncthreads();
+    dv2 acc;
+    acc.x = 0.0;
+    acc.y = 0.0;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll 8
+        for (int j = 0; j < 16; ++j) {
+            dv2 v = buf[t + 256 * j];
+            acc += v;
+            v.y += 1.0;
+            buf[t + 256 * j] = v;
+        }
+    }
+    if (acc.x == -1.0) out[blockIdx.x * 256 + t] = acc.y;  // keeps the loop alive
+}
+
+__global__ void __launch_bounds__(256) copy_probe_kernel(const dv2* __restrict__ src, dv2* __restrict__ dst,
+                                                         uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" {
+
+// Returns GB/s of LDS traffic (bytes read + written) over `blocks`
+// workgroups of 256 threads, or a negative HIP error code.
+double clifft_probe_lds(int blocks, int iters) {
+    double* out = nullptr;
+    if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(double)) != hipSuccess) return -1.0;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipLaunchKernelGGL(lds_probe_kernel, dim3(blocks), dim3(256), 0, 0, out, iters);  // warm-up
+    hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(lds_probe_kernel, dim3(blocks), dim3(256), 0, 0, out, iters);
+    hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(out);
+    if (err != hipSuccess) return -2.0;
+    const double bytes = (double)blocks * 256.0 * 16.0 * 32.0 * (double)iters;  // 16 B read + 16 B write
+    return bytes / (ms * 1e-3) / 1e9;
+}
+
+// Returns GB/s (read + write) of a device-to-device copy of `bytes`.
+double clifft_probe_copy(uint64_t bytes, int reps) {
+    const uint64_t n = bytes / 16;
+    dv2 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return -1.0;
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        hipFree(a);
+        return -1.0;
+    }
+    hipMemset(a, 0, n * 16);
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const dim3 grid(cus * 8), block(256);
+    hipLaunchKernelGGL(copy_probe_kernel, grid, block, 0, 0, a, b, n);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(copy_probe_kernel, grid, block, 0, 0, a, b, n);
+    hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (err != hipSuccess) return -2.0;
+    return 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+}
+
+}  // extern "C"

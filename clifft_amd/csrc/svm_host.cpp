@@ -650,7 +650,7 @@ int clifft_svm_program_destroy(clifft_svm_program* p) {
 int clifft_svm_program_info_get(const clifft_svm_program* p, clifft_svm_program_info* info) {
     if (!p || !info) return fail(CLIFFT_E_INVALID, "null argument");
     info->peak_rank = p->peak_rank;
-    info->tier = p->base_shape.global_amps ? 1u : 0u;
+    info->tier = p->base_shape.lane ? 2u : (p->base_shape.global_amps ? 1u : 0u);
     info->team_waves = p->base_shape.block_team ? p->base_shape.block / 64 : 1;
     info->shots_per_block = p->base_shape.teams_per_block();
     info->num_array_ops = p->num_array_ops;

@@ -280,8 +280,9 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
 int clifft_svm_program_destroy(clifft_svm_program* prog);
 
 /* Static per-program facts the launch planner derived: peak_rank, the tier
- * chosen (0 = LDS-resident shot, 1 = global-resident shot, 2 = device-wide
- * sweep), the number of array instructions and the algorithmic bytes of one
+ * chosen (0 = one shot per wavefront/workgroup team with LDS amplitudes,
+ * 1 = one shot per workgroup team with global amplitudes, 2 = one shot per
+ * lane), the number of array instructions and the algorithmic bytes of one
  * shot (sum over array instructions of their byte model at the static k). */
 typedef struct clifft_svm_program_info {
     uint32_t peak_rank;
