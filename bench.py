@@ -48,7 +48,7 @@ WORKLOADS = {
 
 def build_program(name):
     from clifft_amd import workloads as W
-    from clifft_amd.compile import compile as ccompile
+    from clifft_amd.compiler import compile as ccompile
     if name == "surface_d3":
         return ccompile(W.surface_code_d3())
     if name == "clifford_t20":
@@ -135,7 +135,7 @@ def run_grad(args):
     import torch
     from clifft_amd import _lib, grad
     from clifft_amd import workloads as W
-    from clifft_amd.compile import compile as ccompile
+    from clifft_amd.compiler import compile as ccompile
 
     desc, B = WORKLOADS["rz24_grad"]
     if args.shots:
@@ -225,7 +225,7 @@ def run_batch(args):
     from clifft_amd import _lib, svm
     from clifft_amd import batch as B
     from clifft_amd import workloads as W
-    from clifft_amd.compile import compile as ccompile
+    from clifft_amd.compiler import compile as ccompile
 
     desc, shots = WORKLOADS["batch4096"]
     count = int(os.environ.get("CLIFFT_BENCH_BATCH_COUNT", "4096"))

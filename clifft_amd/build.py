@@ -59,7 +59,7 @@ def build_probe() -> str:
 def build_all() -> None:
     build_svm()
     build_probe()
-    from clifft_amd import compile as _compile
+    from clifft_amd import compiler as _compile
     _compile.build()
 
 

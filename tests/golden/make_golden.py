@@ -40,7 +40,7 @@ def circuits():
 
 
 def main():
-    from clifft_amd.compile import compile as cc
+    from clifft_amd.compiler import compile as cc
     from gradprog import oracle_exp_vals, oracle_param_shift
     from oracle import oracle as orc
     for name, (text, opts) in circuits().items():

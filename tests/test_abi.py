@@ -35,7 +35,7 @@ def test_svm_library_exports_every_declared_symbol():
 
 
 def test_compile_library_exports_every_declared_symbol():
-    from clifft_amd import compile as C
+    from clifft_amd import compiler as C
     lib = C.lib()
     decl = declared_functions("clifft_compile.h")
     missing = [n for n in sorted(decl) if not hasattr(lib, n)]

@@ -17,13 +17,13 @@ import numpy as np
 import pytest
 
 import dense_circuit as DC
-from clifft_amd.compile import compile as ccompile
+from clifft_amd.compiler import compile as ccompile
 from clifft_amd.program import Op
 
 
 @pytest.fixture(scope="module", autouse=True)
 def _built():
-    from clifft_amd import compile as c
+    from clifft_amd import compiler as c
     c.build()
 
 
@@ -144,7 +144,7 @@ def tile_circuit(rng, noisy):
 
 def _tile_text(seed, noisy, tail):
     """First circuit of the seeded family whose compiled form has a U4 node."""
-    from clifft_amd.compile import compile as cc
+    from clifft_amd.compiler import compile as cc
     from clifft_amd.program import Op
     rng = np.random.default_rng(seed)
     for _ in range(50):
@@ -156,7 +156,7 @@ def _tile_text(seed, noisy, tail):
 
 @pytest.mark.parametrize("seed", range(6))
 def test_tile_fusion_matches_dense_and_unfused(orc, seed):
-    from clifft_amd.compile import compile as cc
+    from clifft_amd.compiler import compile as cc
     from dense_circuit import run
     text = _tile_text(seed, False, "EXP_VAL X0*X1 Z0 Y1 Z0*Z1 X0*Y1*Z2")
     _, _, _, e, _ = orc.sample(cc(text), 1, seed=0)

@@ -25,7 +25,7 @@ def load(name):
 
 @pytest.mark.parametrize("name", NAMES)
 def test_compiler_reproduces_blob(name):
-    from clifft_amd.compile import compile as cc
+    from clifft_amd.compiler import compile as cc
     text, prog, _ = load(name)
     kw = {"keep_rotations": True} if name == "rz24" else {}
     assert cc(text, **kw).to_bytes() == prog.to_bytes()

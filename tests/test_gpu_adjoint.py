@@ -27,7 +27,7 @@ def announce(*a):
 @pytest.fixture(scope="module")
 def mods():
     from clifft_amd import _lib, grad
-    from clifft_amd.compile import compile as cc
+    from clifft_amd.compiler import compile as cc
     assert _lib.backend() == "hip:gfx950", _lib.backend()
     return grad, cc
 

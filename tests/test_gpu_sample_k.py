@@ -46,7 +46,7 @@ def test_sample_k_surface_code_uniform(svm, orc, k):
     """All fault sites of the surface-code workload have p = 1e-3, so the
     uniform sampler runs."""
     from clifft_amd import workloads as W
-    from clifft_amd.compile import compile as cc
+    from clifft_amd.compiler import compile as cc
     print("CASE sample_k surface", k, flush=True)
     p = cc(W.surface_code_d3())
     probs = orc.noise_site_probabilities(p)

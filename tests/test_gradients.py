@@ -16,7 +16,7 @@ from gradprog import oracle_exp_vals, oracle_objective, oracle_param_shift, rz_c
 
 @pytest.fixture(scope="module")
 def cc():
-    from clifft_amd.compile import compile as c
+    from clifft_amd.compiler import compile as c
     return c
 
 
