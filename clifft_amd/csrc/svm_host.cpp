@@ -324,11 +324,20 @@ size_t rec_stride_of(const clifft_svm_program* p, bool kfault = false) {
 }
 
 // Highest peak rank run one shot per lane (CLIFFT_LANE_MAX_RANK overrides;
-// a negative value turns the lane tier off).
+// a negative value turns the lane tier off).  Measured on the 20-qubit,
+// peak-6 Clifford+T workload: lanes 1.5e8 shots/s, wavefront teams 1.3e7.
 int lane_max_rank() {
     static int v = [] {
         const char* e = std::getenv("CLIFFT_LANE_MAX_RANK");
-        return e ? std::atoi(e) : 3;
+        return e ? std::atoi(e) : 6;
+    }();
+    return v;
+}
+
+size_t global_workset_bytes() {
+    static size_t v = [] {
+        const char* e = std::getenv("CLIFFT_GLOBAL_WORKSET_MIB");
+        return (size_t)(e ? std::atoll(e) : 256) << 20;
     }();
     return v;
 }
@@ -431,6 +440,14 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStr
         size_t per_team = (size_t)16 << p->peak_rank;
         uint64_t max_teams = std::max<uint64_t>(1, budget / per_team);
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, max_teams / tpb));
+        // Keep the states in flight within the last-level cache budget
+        // (CLIFFT_GLOBAL_WORKSET_MIB, default 256 MiB; 0 = no cap) so each
+        // shot's repeated sweeps hit cache, but never below one team per CU.
+        const size_t ws = global_workset_bytes();
+        if (ws > 0) {
+            const uint64_t by_ws = std::max<uint64_t>((uint64_t)di.num_cus, ws / per_team);
+            cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, by_ws / tpb));
+        }
     }
     L.grid = (uint32_t)std::max<uint64_t>(1, std::min(blocks_needed, cap));
     *gscratch = nullptr;
@@ -1093,7 +1110,7 @@ int batch_check_tuning(const clifft_svm_batch_tuning& t) {
         return fail(CLIFFT_E_INVALID, "batch tuning: need wave_rank_max <= lds_rank_max, wave_rank_max <= 8, lds_rank_max <= 13");
     if (t.wave_chunk == 0 || t.block_chunk == 0 || t.lane_chunk == 0)
         return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
-    if (t.lane_rank_max > 5) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 5");
+    if (t.lane_rank_max > 6) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 6");
     return CLIFFT_OK;
 }
 
@@ -1116,6 +1133,8 @@ int batch_plan(clifft_svm_batch* b, uint64_t cap, hipStream_t st) {
     for (size_t i = 0; i < b->progs.size(); ++i) {
         clifft_svm_program* p = b->progs[i];
         uint32_t tier = tier_of(p, t);
+        if (tier == TIER_LANE && lds_need(p, tier, block) > kBatchMaxDynLds)  // lanes do not fit: next tier
+            tier = p->peak_rank <= t.wave_rank_max ? TIER_WAVE : (p->peak_rank <= t.lds_rank_max ? TIER_LDS : TIER_GLOBAL);
         if (tier != TIER_GLOBAL && lds_need(p, tier, block) > kBatchMaxDynLds) tier = TIER_GLOBAL;
         lds = std::max(lds, lds_need(p, tier, block));
         if (tier == TIER_GLOBAL) {
@@ -1208,7 +1227,7 @@ clifft_svm_batch_tuning default_tuning() {
     t.wave_chunk = 32;
     t.block_chunk = 8;
     t.grid = 0;
-    t.lane_rank_max = lane_max_rank() < 0 ? -1 : std::min(lane_max_rank(), 5);
+    t.lane_rank_max = lane_max_rank() < 0 ? -1 : std::min(lane_max_rank(), 6);
     t.lane_chunk = 1024;
     return t;
 }
@@ -1441,13 +1460,14 @@ int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stre
             }
         }
     };
-    if (!rc) try_lane({-1, 2, 4});
+    if (!rc) try_lane({-1, 2, 4, 6});
     if (!rc) try_knob(&clifft_svm_batch_tuning::lane_chunk, {256, 512, 2048});
     if (!rc) try_knob(&clifft_svm_batch_tuning::lds_rank_max, {9, 10, 12, 13});
     if (!rc) try_knob(&clifft_svm_batch_tuning::wave_rank_max, {5, 6, 8});
     if (!rc) try_knob(&clifft_svm_batch_tuning::block_threads, {512});
     if (!rc) try_knob(&clifft_svm_batch_tuning::wave_chunk, {8, 16, 64, 128});
     if (!rc) try_knob(&clifft_svm_batch_tuning::block_chunk, {2, 4, 16, 32});
+    if (!rc) try_knob(&clifft_svm_batch_tuning::grid, {256, 512, 768});
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;

@@ -73,18 +73,28 @@ class Adjoint:
 
 
 def _run_torch(adj: Adjoint, thetas, coeffs, want_grad: bool):
+    """Launch on a side stream ordered after, and before, torch's current
+    stream (which may be the null stream, a handle the C ABI reads as "the
+    library's own stream")."""
     import torch
-    th = thetas.detach().to(torch.float64).contiguous()
-    B = th.shape[0]
-    ev = torch.empty((B, adj.num_exp_vals), dtype=torch.float64, device=th.device)
-    g = torch.empty((B, adj.num_params), dtype=torch.float64, device=th.device) if want_grad else None
-    c = coeffs.detach().to(device=th.device, dtype=torch.float64).contiguous()
-    stream = torch.cuda.current_stream(th.device).cuda_stream
-    _lib.check(_lib.lib().clifft_svm_adjoint_run(adj.handle, th.data_ptr() if th.numel() else None, B,
-                                                 c.data_ptr() if c.numel() else None,
-                                                 ev.data_ptr() if ev.numel() else None,
-                                                 None if g is None else g.data_ptr(), _lib.REC_DEVICE, stream),
-               "adjoint_run")
+    cur = torch.cuda.current_stream(thetas.device)
+    side = torch.cuda.Stream(device=thetas.device)
+    side.wait_stream(cur)
+    with torch.cuda.stream(side):
+        th = thetas.detach().to(torch.float64).contiguous()
+        B = th.shape[0]
+        ev = torch.empty((B, adj.num_exp_vals), dtype=torch.float64, device=th.device)
+        g = torch.empty((B, adj.num_params), dtype=torch.float64, device=th.device) if want_grad else None
+        c = coeffs.detach().to(device=th.device, dtype=torch.float64).contiguous()
+        _lib.check(_lib.lib().clifft_svm_adjoint_run(adj.handle, th.data_ptr() if th.numel() else None, B,
+                                                     c.data_ptr() if c.numel() else None,
+                                                     ev.data_ptr() if ev.numel() else None,
+                                                     None if g is None else g.data_ptr(), _lib.REC_DEVICE,
+                                                     side.cuda_stream),
+                   "adjoint_run")
+    cur.wait_stream(side)
+    for t in (th, ev, c) + ((g,) if g is not None else ()):
+        t.record_stream(cur)
     return ev, g, c
 
 

@@ -377,7 +377,7 @@ typedef struct {
     uint32_t block_chunk;   /* shots per work item, workgroup tiers */
     uint32_t grid;          /* workgroups (0 = resident capacity of the device) */
     int32_t lane_rank_max;  /* <= 64-qubit programs up to this rank run one shot
-                               per lane (-1 = never), <= 5 */
+                               per lane (-1 = never), <= 6 */
     uint32_t lane_chunk;    /* shots per work item, lane tier */
 } clifft_svm_batch_tuning;
 

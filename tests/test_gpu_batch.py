@@ -110,6 +110,7 @@ TUNINGS = [dict(block_threads=512), dict(lds_rank_max=9), dict(lds_rank_max=13),
            dict(wave_rank_max=8, lds_rank_max=8), dict(wave_chunk=4, block_chunk=1), dict(grid=7),
            dict(block_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32),
            dict(lane_rank_max=-1), dict(lane_rank_max=5, lane_chunk=100), dict(lane_rank_max=1, lane_chunk=64),
+           dict(lane_rank_max=6, lane_chunk=256),
            dict(block_threads=512, lane_rank_max=4, lane_chunk=3000)]
 
 
@@ -199,6 +200,6 @@ def test_batch_rejects_bad_tuning(mods):
     with pytest.raises(SvmError):
         b.set_tuning(lds_rank_max=14)
     with pytest.raises(SvmError):
-        b.set_tuning(lane_rank_max=6)
+        b.set_tuning(lane_rank_max=7)
     b.run(seeds)  # still usable with the previous tuning
     b.close()

@@ -42,16 +42,25 @@ def test_sample_k_random_programs(svm, orc, nq, peak, k):
 
 
 @pytest.mark.parametrize("k", [1, 2, 4])
-def test_sample_k_surface_code_uniform(svm, orc, k):
-    """All fault sites of the surface-code workload have p = 1e-3, so the
-    uniform sampler runs."""
+def test_sample_k_surface_code(svm, orc, k):
+    """The surface-code workload (its DEPOLARIZE2 channel sums differ from
+    p = 1e-3 in the last bit, so the DP sampler runs)."""
     from clifft_amd import workloads as W
     from clifft_amd.compiler import compile as cc
     print("CASE sample_k surface", k, flush=True)
     p = cc(W.surface_code_d3())
-    probs = orc.noise_site_probabilities(p)
-    assert np.all(probs == probs[0])
     same(svm.sample_k(p, 2000, k, seed=3), orc.sample_k(p, 2000, k, seed=3))
+
+
+@pytest.mark.parametrize("k", [1, 3, 6])
+def test_sample_k_uniform_sampler(svm, orc, k):
+    """Equal probabilities on every site: the partial Fisher-Yates sampler."""
+    print("CASE sample_k uniform", k, flush=True)
+    p = fault_program([0.03] * 7, readout=[0.03] * 3)
+    assert np.all(orc.noise_site_probabilities(p) == 0.03)
+    res = svm.sample_k(p, 3000, k, seed=k + 1)
+    assert np.all(res.measurements.sum(axis=1) == k)
+    same(res, orc.sample_k(p, 3000, k, seed=k + 1))
 
 
 def test_sample_k_fault_sets(svm, orc):
