@@ -238,3 +238,38 @@ def random_measure_only_program(seed, num_qubits=8, peak=5, length=120):
     record_probabilities: no noise, detectors, observables)."""
     return random_program(seed, num_qubits, peak, length, noise=False, exp_vals=True, readout=False,
                           postselect=False, u_nodes=True, classical=False)
+
+
+def fanout_program(seed, peak, num_qubits=None, frames=True):
+    """Expansions to rank `peak`, then runs of ARRAY_CNOTs sharing a control
+    (the device fuses each run into one fan-out sweep), mixed with H/T, noise
+    on the frame, EXP_VAL probes and final measurements of every axis."""
+    from clifft_amd.program import (Op, build_program, make_array_cnot, make_array_h, make_array_t,
+                                    make_exp_val, make_expand_t, make_frame_h, make_meas, make_noise)
+    rng = np.random.default_rng(seed)
+    n = num_qubits or peak + 2
+    bc = []
+    for v in range(peak):
+        bc += [make_frame_h(v), make_expand_t(v)]
+    sites = []
+    for _ in range(6):
+        c = int(rng.integers(0, peak))
+        targets = [int(t) for t in rng.permutation(peak) if t != c][: int(rng.integers(1, max(2, peak)))]
+        for t in targets:
+            bc.append(make_array_cnot(c, t))
+        if rng.random() < 0.3 and targets:
+            bc.append(make_array_cnot(c, targets[0]))  # repeated target ends the run
+        bc.append(make_array_h(int(rng.integers(0, peak))))
+        bc.append(make_array_t(int(rng.integers(0, peak))))
+        if frames:
+            sites.append([(int(rng.integers(1, 1 << peak)), int(rng.integers(0, 1 << peak)), 0.3)])
+            bc.append(make_noise(len(sites) - 1))
+    ev_masks = [(int(rng.integers(0, 1 << peak)), int(rng.integers(0, 1 << peak)), False) for _ in range(3)]
+    for i in range(3):
+        bc.append(make_exp_val(i, i))
+    k = peak
+    for j in range(peak):
+        k -= 1
+        bc.append(make_meas(Op.MEAS_ACTIVE_INTERFERE if j % 2 else Op.MEAS_ACTIVE_DIAGONAL, k, j))
+    return build_program(bc, peak_rank=peak, num_qubits=n, num_measurements=peak, num_exp_vals=3,
+                         exp_val_masks=ev_masks, noise_sites=sites)

@@ -132,3 +132,18 @@ def test_reference_stream_first_shot(svm, orc, peak):
     m, d, o, e, _ = orc.sample(p, 1, seed=1234, per_shot=False)
     res = svm.sample(p, 1, seed=1234)
     assert np.array_equal(res.measurements, m) and np.array_equal(res.detectors, d)
+
+
+@pytest.mark.parametrize("peak", [3, 5, 9, 15])
+def test_fanout_cnot_runs(svm, orc, peak):
+    """Runs of CNOTs sharing a control execute as one fan-out sweep on the
+    device (lane and team tiers) and one CNOT at a time in the oracle."""
+    from randprog import fanout_program
+    announce("fanout", peak)
+    for seed in range(3):
+        p = fanout_program(seed, peak)
+        shots = 64 if peak >= 12 else 400
+        res = svm.sample(p, shots, seed=seed)
+        m, d, o, e, _ = orc.sample(p, shots, seed=seed, per_shot=True)
+        assert np.array_equal(res.measurements, m)
+        np.testing.assert_allclose(res.exp_vals, e, atol=ATOL, rtol=0)
