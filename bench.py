@@ -221,21 +221,15 @@ def run_grad(args):
     print(json.dumps(line), flush=True)
 
 
-def run_batch(args):
-    """Config: 4096 programs x 1000 shots in one persistent launch (autotuned),
-    against one plain launch per program on the same stream."""
+def _measure_batch(progs, shots, seeds0, steps, warmup):
+    """One persistent launch (autotuned) vs one plain launch per program, on
+    the same stream and outputs.  Returns timings in seconds."""
+    import ctypes
+
     import torch
     from clifft_amd import _lib, svm
     from clifft_amd import batch as B
-    from clifft_amd import workloads as W
-    from clifft_amd.compiler import compile as ccompile
-
-    desc, shots = WORKLOADS["batch4096"]
-    count = int(os.environ.get("CLIFFT_BENCH_BATCH_COUNT", "4096"))
-    if args.shots:
-        shots = args.shots
-    progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
-    seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
+    count = len(progs)
     dev = torch.device("cuda:0")
     nb = lambda n: (n + 7) // 8
     bufs, outs = [], []
@@ -249,10 +243,8 @@ def run_batch(args):
     bt = B.Batch(progs, [shots] * count, outs, packed=True)
     tuning = bt.tune(sample_shots=max(1, shots // 8), stream=sh)
     torch.cuda.synchronize()
-
-    # per-program plain launches (the unbatched baseline), same outputs
     recs = (_lib.Records * count)(*[_lib.Records(o["measurements"], None, None, None, None) for o in outs])
-    harr = (__import__("ctypes").c_void_p * count)(*[h.handle.value for h in handles])
+    harr = (ctypes.c_void_p * count)(*[h.handle.value for h in handles])
     shots_arr = np.full(count, shots, dtype=np.uint64)
     flags_pp = _lib.REC_DEVICE | _lib.REC_PACKED | _lib.BATCH_PER_PROGRAM
 
@@ -262,21 +254,20 @@ def run_batch(args):
 
     per_program(seeds0)
     torch.cuda.synchronize()
+    reps_pp = max(1, min(3, steps))
     t0 = time.perf_counter()
-    reps_pp = max(1, min(3, args.steps))
     for r in range(reps_pp):
         per_program(seeds0 + np.uint64(r + 1))
     torch.cuda.synchronize()
     t_pp = (time.perf_counter() - t0) / reps_pp
-
-    for i in range(args.warmup):
+    for i in range(warmup):
         bt.run(seeds0 + np.uint64(100 + i), stream=sh)
     torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     with torch.cuda.stream(stream):
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             starts[i].record(stream)
             bt.run(seeds0 + np.uint64(1000 + i), stream=sh)
             ends[i].record(stream)
@@ -285,34 +276,61 @@ def run_batch(args):
         bt.run_graph(seeds0, stream=sh)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             bt.run_graph(seeds0 + np.uint64(2000 + i), stream=sh)
         torch.cuda.synchronize()
         wall_graph = time.perf_counter() - t0
     kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-    total = shots * count
-    value = total * args.steps / wall
     alg = sum((h.info()["bytes_per_shot"] + nb(p.num_measurements)) * shots for h, p in zip(handles, progs))
-    achieved = alg / (kern_ms * 1e-3) / 1e9
+    bt.close()
+    return {"t_batch": wall / steps, "t_graph": wall_graph / steps, "t_per_program": t_pp, "kern_ms": kern_ms,
+            "alg_bytes": alg, "tuning": tuning}
+
+
+def run_batch(args):
+    """Config: 4096 programs x 1000 shots in one persistent launch (autotuned),
+    against one plain launch per program on the same stream.  Also reported:
+    the same comparison on the programs of peak rank <= 13 alone (the part
+    that one plain launch per program cannot fill the chip with)."""
+    from clifft_amd import _lib
+    from clifft_amd import workloads as W
+    from clifft_amd.compiler import compile as ccompile
+
+    desc, shots = WORKLOADS["batch4096"]
+    count = int(os.environ.get("CLIFFT_BENCH_BATCH_COUNT", "4096"))
+    if args.shots:
+        shots = args.shots
+    progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
+    seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
+    full = _measure_batch(progs, shots, seeds0, args.steps, args.warmup)
+    small = [i for i, p in enumerate(progs) if p.peak_rank <= 13]
+    sub = _measure_batch([progs[i] for i in small], shots, seeds0[small], args.steps, args.warmup)
+    total = shots * count
+    value = total / full["t_batch"]
+    achieved = full["alg_bytes"] / (full["kern_ms"] * 1e-3) / 1e9
     peaks = np.bincount([p.peak_rank for p in progs], minlength=17)
     line = {
         "metric": "shots/s", "value": value, "unit": "shots/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": full["t_batch"] * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batch4096", "description": desc, "programs": count, "shots_per_program": shots,
                    "peak_rank_histogram": peaks.tolist(),
                    "mean_instructions": float(np.mean([len(p.bytecode) for p in progs])),
-                   "tuning": tuning, "records": "bit-packed measurements, device memory",
-                   "per_program_launches_shots_per_s": total / t_pp,
-                   "speedup_vs_per_program": (total / (wall / args.steps)) / (total / t_pp),
-                   "graph_replay_shots_per_s": total * args.steps / wall_graph,
+                   "tuning": full["tuning"], "records": "bit-packed measurements, device memory",
+                   "per_program_launches_shots_per_s": total / full["t_per_program"],
+                   "speedup_vs_per_program": full["t_per_program"] / full["t_batch"],
+                   "graph_replay_shots_per_s": total / full["t_graph"],
+                   "peak_le_13_subset": {"programs": len(small),
+                                         "batch_shots_per_s": shots * len(small) / sub["t_batch"],
+                                         "per_program_shots_per_s": shots * len(small) / sub["t_per_program"],
+                                         "speedup_vs_per_program": sub["t_per_program"] / sub["t_batch"],
+                                         "tuning": sub["tuning"]},
                    "parallelism": "programs x shots"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kern_ms,
-                     "algorithmic_bytes_per_launch": alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": full["kern_ms"],
+                     "algorithmic_bytes_per_launch": full["alg_bytes"]},
         "backend": _lib.backend(),
     }
-    bt.close()
     print(json.dumps(line), flush=True)
 
 
