@@ -40,7 +40,8 @@ WORKLOADS = {
     "clifford_t20": ("20 qubits x 40 layers random Clifford+T, 6 T, final M (peak k=6)", 100_000),
     "large_k34": ("34 qubits x 40 layers, 26 T on distinct qubits, MR on 4 qubits / 10 layers", 256),
     "clifford_t24_k16": ("24 qubits x 40 layers random Clifford+T, 16 T, final M (peak k=16)", 2000),
-    "rz24_grad": ("24 qubits, 30 Clifford layers, 16 R_Z(theta_i) slots (peak k=16), 32 random Pauli probes; "
+    "rz24_grad": ("24 qubits, 30 Clifford layers, 16 R_Z(theta_i) slots (peak k=16), 32 random Pauli probes "
+                  "with nonzero expectation; "
                   "<O> and d<O>/dtheta for 512 theta vectors by the adjoint kernel", 512),
     "batch4096": ("4096 random Clifford+T programs, n ~ U[16,40], T-count ~ U[4,16], 40 layers, final M; "
                   "one persistent launch", 1000),

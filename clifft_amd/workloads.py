@@ -13,6 +13,7 @@
                         T-count ~ U[4,16]
     rz_exp_val          24 qubits, 30 Clifford layers interleaved with 16
                         R_Z(theta/pi) rotations, 32 random Pauli-string probes
+                        (drawn among the strings with a nonzero expectation)
 """
 
 from __future__ import annotations
@@ -140,13 +141,24 @@ def batch_programs(count=4096, seed=42, shots=1000):
     return out
 
 
-def rz_exp_val(n=24, layers=30, n_params=16, n_terms=32, seed=42):
+def rz_exp_val(n=24, layers=30, n_params=16, n_terms=32, seed=42, nonvanishing=True):
     """Returns (circuit_text_builder(thetas) -> text, coeffs, pauli_terms).
 
     The circuit has n_params R_Z slots whose angles are filled in by the
     builder (theta in radians -> R_Z(theta / pi) in half-turns); the probes
     are `n_terms` random Pauli strings (each qubit I/X/Y/Z uniformly), with
-    coefficients ~ N(0, 1) for the observable sum."""
+    coefficients ~ N(0, 1) for the observable sum.
+
+    nonvanishing: draw the strings uniformly among those whose expectation
+    is not identically zero in theta.  After the Clifford layers the state
+    is a product of the n_params rotated axes (each (|0> + z|1>)/sqrt 2 in
+    the compiled frame) and a stabilizer state on the dormant axes, so a
+    string is kept when its compiled image has X or Y (not Z alone) on the
+    active axes it touches, at least one of them, and no X support on a
+    dormant axis.  An unconditioned draw has <P> = 0 exactly for every probe
+    of the seed-42 draw at n = 24, which would make the workload vacuous.
+    The circuit-frame string of a chosen image is found by inverting the
+    compiled images of X_q and Z_q over GF(2)."""
     rng = np.random.default_rng(seed)
     param_layers = sorted(rng.choice(layers, size=n_params, replace=n_params > layers))
     param_qubits = [int(rng.integers(0, n)) for _ in range(n_params)]
@@ -155,15 +167,8 @@ def rz_exp_val(n=24, layers=30, n_params=16, n_terms=32, seed=42):
         ones = rng.random(n) < 0.5
         perm = rng.permutation(n)
         layer_ops.append((ones, perm))
-    terms = []
-    while len(terms) < n_terms:
-        letters = rng.integers(0, 4, size=n)
-        t = "*".join(f"{'IXYZ'[l]}{q}" for q, l in enumerate(letters) if l)
-        if t:
-            terms.append(t)
-    coeffs = rng.normal(size=n_terms)
 
-    def build(thetas):
+    def body(thetas):
         thetas = np.asarray(thetas, dtype=np.float64)
         lines = []
         pi_idx = 0
@@ -174,7 +179,104 @@ def rz_exp_val(n=24, layers=30, n_params=16, n_terms=32, seed=42):
                 lines.append(f"R_Z({float(thetas[pi_idx] / np.pi):.17g}) {param_qubits[pi_idx]}")
                 pi_idx += 1
             lines.append("CX " + " ".join(f"{a} {b}" for a, b in zip(perm[::2], perm[1::2])))
-        lines.append("EXP_VAL " + " ".join(terms))
-        return "\n".join(l for l in lines if l.split()[1:])
+        return [l for l in lines if l.split()[1:]]
+
+    def draw(count):
+        out = []
+        while len(out) < count:
+            letters = rng.integers(0, 4, size=n)
+            t = "*".join(f"{'IXYZ'[l]}{q}" for q, l in enumerate(letters) if l)
+            if t:
+                out.append(t)
+        return out
+
+    if nonvanishing:
+        terms = _nonvanishing_terms(body(np.full(n_params, 0.25)), n, n_params, n_terms, rng)
+    else:
+        terms = draw(n_terms)
+    coeffs = rng.normal(size=n_terms)
+
+    def build(thetas):
+        return "\n".join(body(thetas) + ["EXP_VAL " + " ".join(terms)])
 
     return build, coeffs, terms
+
+
+def _gf2_solve(M, t):
+    """x with M x = t over GF(2) (M square, invertible; uint8 0/1)."""
+    n = M.shape[0]
+    A = np.concatenate([M.copy(), t.reshape(-1, 1)], axis=1) & 1
+    for c in range(n):
+        piv = c + int(np.flatnonzero(A[c:, c])[0])
+        A[[c, piv]] = A[[piv, c]]
+        rows = np.flatnonzero(A[:, c])
+        rows = rows[rows != c]
+        A[rows] ^= A[c]
+    return A[:, n]
+
+
+def _nonvanishing_terms(lines, n, k, count, rng):
+    """`count` circuit-frame Pauli strings whose compiled images act as X/Y
+    (or I) on the k active axes, touching at least one, and as Z/I on the
+    dormant ones."""
+    from .compiler import compile as ccompile
+    basis = [f"{c}{q}" for q in range(n) for c in "XZ"]
+    p = ccompile("\n".join(lines + ["EXP_VAL " + " ".join(basis)]), keep_rotations=True)
+    w = p.words
+    ex = np.asarray(p.ev_x).reshape(-1, w)
+    ez = np.asarray(p.ev_z).reshape(-1, w)
+    order = [int(ins["idx0"]) for ins in p.bytecode if int(ins["opcode"]) == _op_exp_val()]
+    if len(order) != 2 * n:
+        raise RuntimeError("basis probes changed in compilation")
+
+    def bits(words):
+        v = sum(int(x) << (64 * i) for i, x in enumerate(words))
+        return [(v >> a) & 1 for a in range(n)]
+
+    M = np.zeros((2 * n, 2 * n), np.uint8)
+    for col, mi in enumerate(order):
+        M[:n, col] = bits(ex[mi])
+        M[n:, col] = bits(ez[mi])
+    terms = []
+    while len(terms) < count:
+        act = rng.integers(0, 3, size=k)  # 0 = I, 1 = X, 2 = Y on the active axes
+        if not act.any():
+            continue
+        tx = np.zeros(n, np.uint8)
+        tz = np.zeros(n, np.uint8)
+        tx[:k] = act > 0
+        tz[:k] = act == 2
+        tz[k:] = rng.integers(0, 2, size=n - k)
+        sol = _gf2_solve(M, np.concatenate([tx, tz]))
+        letters = ["IXZY"[int(sol[2 * q]) + 2 * int(sol[2 * q + 1])] for q in range(n)]
+        terms.append("*".join(f"{c}{q}" for q, c in enumerate(letters) if c != "I"))
+    return terms
+
+
+def _op_exp_val():
+    from .program import Op
+    return Op.EXP_VAL
+
+
+def _active_x_only(lines, terms):
+    """For each Pauli string, whether its compiled EXP_VAL mask has X support
+    only on axes active at the probe (static rank from the EXPAND count)."""
+    from .compiler import compile as ccompile
+    from .program import Op
+    p = ccompile("\n".join(lines + ["EXP_VAL " + " ".join(terms)]), keep_rotations=True)
+    bc = p.bytecode
+    expands = (Op.EXPAND, Op.EXPAND_T, Op.EXPAND_T_DAG, Op.EXPAND_ROT)
+    k, ok = 0, []
+    for ins in bc:
+        op = int(ins["opcode"])
+        if op in expands:
+            k += 1
+        elif op == Op.EXP_VAL:
+            words = np.asarray(p.ev_x).reshape(-1, p.words)[int(ins["idx0"])]
+            x = sum(int(v) << (64 * i) for i, v in enumerate(words))
+            ok.append((x >> k) == 0)
+        elif op in (Op.MEAS_ACTIVE_DIAGONAL, Op.MEAS_ACTIVE_INTERFERE, Op.SWAP_MEAS_INTERFERE):
+            k -= 1
+    if len(ok) != len(terms):
+        raise RuntimeError("probe count changed in compilation")
+    return ok

@@ -50,3 +50,4 @@ def test_oracle_reproduces_gradients(orc):
         np.testing.assert_allclose(oracle_exp_vals(orc, prog, g["thetas"][b]), g["exp_vals"][b], atol=1e-13, rtol=0)
         np.testing.assert_allclose(oracle_param_shift(orc, prog, g["thetas"][b], g["coeffs"]), g["grads"][b],
                                    atol=1e-12, rtol=0)
+    assert np.all(np.abs(g["exp_vals"]) > 1e-9) and np.abs(g["grads"]).max() > 1e-2

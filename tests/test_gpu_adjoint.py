@@ -108,6 +108,7 @@ def test_workload_rz24_matches_oracle(mods, orc):
     for b in range(2):
         np.testing.assert_allclose(ev[b], oracle_exp_vals(orc, p, th[b]), atol=1e-12, rtol=0)
         np.testing.assert_allclose(g[b], oracle_param_shift(orc, p, th[b], coeffs), atol=1e-10, rtol=0)
+    assert np.all(np.abs(ev) > 1e-9) and np.abs(g).max() > 1e-2  # the probes are not identically zero
 
 
 def test_adjoint_rejects_measurements(mods):
