@@ -20,6 +20,7 @@ struct LaunchShape {
     bool block_team = false;   // team = whole workgroup (else = one wavefront)
     bool global_amps = false;  // amplitudes in global scratch (else LDS)
     bool lane = false;         // one shot per lane (small rank, <= 64 qubits)
+    bool lane_dense = false;   // lane kernel built for 6 resident workgroups (small LDS columns)
     uint32_t block = 256;      // threads per workgroup
     uint32_t grid = 1;         // workgroups
     size_t lds_bytes = 0;      // dynamic LDS per workgroup

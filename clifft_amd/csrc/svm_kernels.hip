@@ -310,7 +310,11 @@ __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
     }
 }
 
-__global__ void __launch_bounds__(256, 4) lane_kernel(const DevProgram* P, const ShotArgs* A) {
+// MINB = resident workgroups of 256 the register budget is sized for: 4
+// (128 VGPRs, no spills) for programs whose LDS columns limit occupancy
+// anyway, 6 (80 VGPRs, a few spills) when LDS would allow more waves.
+template <int MINB>
+__global__ void __launch_bounds__(256, MINB) lane_kernel(const DevProgram* P, const ShotArgs* A) {
     cptr<ShotArgs> Ag = cp(A);
     const uint64_t wpb = blockDim.x >> 6;
     const uint64_t wave_global = (uint64_t)blockIdx.x * wpb + uni(threadIdx.x >> 6);
@@ -462,15 +466,21 @@ static hipError_t launch_one(const LaunchShape& L, const DevProgram* P, const Sh
     return hipGetLastError();
 }
 
+static const void* lane_fn(const LaunchShape& L) {
+    return L.lane_dense ? reinterpret_cast<const void*>(lane_kernel<6>) : reinterpret_cast<const void*>(lane_kernel<4>);
+}
+
 static hipError_t launch_lane(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
-    static bool attr_done = false;
-    if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(lane_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    static bool attr_done[2] = {false, false};
+    if (!attr_done[L.lane_dense]) {
+        hipError_t e = hipFuncSetAttribute(lane_fn(L), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
-        attr_done = true;
+        attr_done[L.lane_dense] = true;
     }
-    hipLaunchKernelGGL(lane_kernel, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
+    if (L.lane_dense)
+        hipLaunchKernelGGL(lane_kernel<6>, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
+    else
+        hipLaunchKernelGGL(lane_kernel<4>, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
     return hipGetLastError();
 }
 
@@ -491,7 +501,7 @@ static const void* shot_fn_fk(const LaunchShape& L) {
 }
 
 hipError_t shot_occupancy(const LaunchShape& L, int* blocks_per_cu) {
-    const void* fn = L.lane ? reinterpret_cast<const void*>(lane_kernel)
+    const void* fn = L.lane ? lane_fn(L)
                             : (L.nw == 1 ? shot_fn_fk<FRAME_W1>(L) : shot_fn_fk<FRAME_LANES>(L));
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
