@@ -325,6 +325,13 @@ struct LaneInterp {
         cptr<DevProgram> P0 = opaque(Pg);
         cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(P0->bc));
         const uint32_t n_instr = P0->n_instr;
+        // A shot that ends early (post-selection discard, unreachable
+        // forced record) clears `live` and skips the remaining opcodes
+        // instead of leaving the loop: the loop then has no per-lane exit,
+        // so the compiler keeps pc, the opcode fetch and the dispatch on the
+        // scalar unit (a per-lane exit would make them per-lane values,
+        // fetched with vector loads and dispatched under exec masks).
+        uint32_t live = 1;
         for (uint32_t pc = 0; pc < n_instr; ++pc) {
             const uint64_t w0 = code[4 * pc], w1 = code[4 * pc + 1];
             const uint32_t op = (uint32_t)(w0 & 0xFF);
@@ -333,6 +340,7 @@ struct LaneInterp {
             const uint32_t a2 = (uint32_t)(w0 >> 48);
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
+            if (!live) continue;
             switch (op) {
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
                 case CLIFFT_OP_FRAME_CZ: frame_cz(a1, a2); break;
@@ -567,7 +575,8 @@ struct LaneInterp {
                     const uint32_t det_v = (flags & CLIFFT_FLAG_IDENTITY) ? sign : (gx(a1) ^ sign);
                     if ((forced[i0] ? 1u : 0u) != det_v) {
                         reachable = 0;
-                        return;
+                        live = 0;
+                        break;
                     }
                     rset(rm, i0, det_v);
                     break;
@@ -583,13 +592,13 @@ struct LaneInterp {
                     break;
                 }
                 case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL_FORCED:
-                    if (!meas_diagonal(a1, i0, sign, true)) return;
+                    if (!meas_diagonal(a1, i0, sign, true)) live = 0;
                     break;
                 case CLIFFT_OP_MEAS_ACTIVE_INTERFERE_FORCED:
-                    if (!meas_interfere(a1, i0, sign, true)) return;
+                    if (!meas_interfere(a1, i0, sign, true)) live = 0;
                     break;
                 case CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED:
-                    if (!swap_meas_interfere(a1, a2, i0, sign, true)) return;
+                    if (!swap_meas_interfere(a1, a2, i0, sign, true)) live = 0;
                     break;
                 case CLIFFT_OP_APPLY_PAULI:
                     if (rget(rm, i1)) {
@@ -629,7 +638,7 @@ struct LaneInterp {
                         rset(rd, i1, 0);
                         if (par) {
                             discarded = 1;
-                            return;
+                            live = 0;
                         }
                     }
                     break;
