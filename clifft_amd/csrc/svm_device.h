@@ -135,7 +135,22 @@ struct ShotArgs {
     double* preset_ev;                // MODE_PRESET exp_vals
     unsigned long long* dust_counter; // optional telemetry
     const KfTables* kf;               // MODE_KFAULT
+    uint32_t amp_slices;              // global tier: amplitude slices per team (2 = fused monomial runs)
+    uint32_t pad0;
 };
+
+// One deferred permutation/phase opcode of a fused run (global tier): the
+// run's opcodes are applied to every amplitude in one out-of-place pass.
+enum : uint32_t { MONO_CNOT = 0, MONO_CZ = 1, MONO_SWAP = 2, MONO_MULTI_CNOT = 3, MONO_MULTI_CZ = 4, MONO_PHASE = 5 };
+struct MonoOp {
+    uint32_t kind;
+    uint32_t a;  // control / first axis / MULTI axis / phase axis
+    uint32_t b;  // target / second axis
+    uint32_t m;  // MULTI_* mask (active axes only)
+    double pr, pi;  // MONO_PHASE factor
+};
+constexpr uint32_t kMaxMono = 32;
+constexpr uint32_t kMonoBytes = kMaxMono * sizeof(MonoOp);
 
 // Team-tier LDS scratch of a workgroup: the reduction area (16 waves x 4
 // doubles) and, per wavefront, the phases of pending fused expansions

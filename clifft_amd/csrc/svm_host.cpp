@@ -342,6 +342,16 @@ size_t global_workset_bytes() {
     return v;
 }
 
+// Fused permutation/phase runs in the global tier (CLIFFT_FUSE_MONO=0 turns
+// them off).
+bool fuse_mono_enabled() {
+    static bool v = [] {
+        const char* e = std::getenv("CLIFFT_FUSE_MONO");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return v;
+}
+
 // Choose the team shape and storage tier for a program's peak rank.
 LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool kfault = false) {
     LaunchShape L;
@@ -386,6 +396,13 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
     if (lds > 160 * 1024) {  // records too large next to LDS amplitudes
         L.global_amps = true;
         lds = red + waves * rec_stride;
+    }
+    // Workgroup teams on global amplitudes get a second slice and an LDS
+    // list for fused permutation/phase runs (one read and one write of the
+    // state per run instead of one pass per opcode).
+    if (L.global_amps && L.block_team && fuse_mono_enabled() && lds + kMonoBytes <= 160 * 1024) {
+        L.amp_slices = 2;
+        lds += kMonoBytes;
     }
     L.lds_bytes = lds;
     return L;
@@ -442,7 +459,7 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStr
         size_t headroom = std::max<size_t>(budget * 2 / 5, 16ull << 30);
         budget = budget > headroom ? budget - headroom : (size_t)1 << 30;
         budget = std::min<size_t>(budget, 192ull << 30);
-        size_t per_team = (size_t)16 << p->peak_rank;
+        size_t per_team = ((size_t)16 << p->peak_rank) * L.amp_slices;
         uint64_t max_teams = std::max<uint64_t>(1, budget / per_team);
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, max_teams / tpb));
         // Keep the states in flight within the last-level cache budget
@@ -457,7 +474,7 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStr
     L.grid = (uint32_t)std::max<uint64_t>(1, std::min(blocks_needed, cap));
     *gscratch = nullptr;
     if (L.global_amps) {
-        size_t need = (size_t)L.grid * tpb * ((size_t)16 << p->peak_rank);
+        size_t need = (size_t)L.grid * tpb * ((size_t)16 << p->peak_rank) * L.amp_slices;
         int rc = ensure_scratch(st, need, gscratch);
         if (rc) return rc;
     }
@@ -470,6 +487,7 @@ void fill_common_args(clifft_svm_program* p, ShotArgs& A, void* gscratch, bool k
     A.amp_elems = 1u << p->peak_rank;
     A.gscratch = reinterpret_cast<dv2*>(gscratch);
     A.dust_counter = p->dust_dev;
+    A.amp_slices = 1;
 }
 
 size_t row_bytes(uint32_t n, bool packed) { return packed ? (n + 7) / 8 : n; }
@@ -700,6 +718,7 @@ static int run_shots(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint3
         if (rc) return rc;
         ShotArgs A;
         fill_common_args(p, A, gs, kfault);
+        A.amp_slices = L.global_amps ? L.amp_slices : 1u;
         A.kf = kf;
         A.shot_begin = base;
         A.shot_count = n;
@@ -1004,6 +1023,7 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
     if (rc) return rc;
     ShotArgs A;
     fill_common_args(p, A, gs);
+    A.amp_slices = L.global_amps ? L.amp_slices : 1u;
     A.shot_begin = 0;
     A.shot_count = 1;
     A.mode = MODE_PRESET;

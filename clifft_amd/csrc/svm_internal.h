@@ -21,6 +21,7 @@ struct LaunchShape {
     bool global_amps = false;  // amplitudes in global scratch (else LDS)
     bool lane = false;         // one shot per lane (small rank, <= 64 qubits)
     bool lane_dense = false;   // lane kernel built for 6 resident workgroups (small LDS columns)
+    uint32_t amp_slices = 1;   // global-amplitude slices per team (2: fused permutation/phase runs)
     uint32_t block = 256;      // threads per workgroup
     uint32_t grid = 1;         // workgroups
     size_t lds_bytes = 0;      // dynamic LDS per workgroup

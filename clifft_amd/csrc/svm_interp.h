@@ -111,6 +111,11 @@ struct Interp {
     uint32_t kf;
     const __attribute__((address_space(3))) uint32_t* kf_list;
     uint32_t kf_nn, kf_npos, kf_nr, kf_rpos;
+    // Global-tier workgroup teams: the team's second amplitude slice and the
+    // LDS list of deferred permutation/phase opcodes (null: no fusion).
+    AmpT* amp_alt;
+    lds_mono* mops;
+    uint32_t nmono;
 
     // ----------------------------------------------------------------- gamma
     CSVM_INLINE void mul_phase(double re, double im) {
@@ -140,12 +145,65 @@ struct Interp {
         (1ull << CLIFFT_OP_NOISE_BLOCK) | (1ull << CLIFFT_OP_READOUT_NOISE) | (1ull << CLIFFT_OP_DETECTOR) |
         (1ull << CLIFFT_OP_OBSERVABLE);
 
+    // Permutation/phase opcodes that a fused run may hold: runs of them
+    // (with kNoAmpOps other than EXPAND* in between) become one pass.
+    static constexpr uint64_t kMonoOps =
+        (1ull << CLIFFT_OP_ARRAY_CNOT) | (1ull << CLIFFT_OP_ARRAY_CZ) | (1ull << CLIFFT_OP_ARRAY_SWAP) |
+        (1ull << CLIFFT_OP_ARRAY_MULTI_CNOT) | (1ull << CLIFFT_OP_ARRAY_MULTI_CZ) | (1ull << CLIFFT_OP_ARRAY_S) |
+        (1ull << CLIFFT_OP_ARRAY_S_DAG) | (1ull << CLIFFT_OP_ARRAY_T) | (1ull << CLIFFT_OP_ARRAY_T_DAG) |
+        (1ull << CLIFFT_OP_ARRAY_ROT);
+    static constexpr uint64_t kMonoKeepOps =
+        kMonoOps | (kNoAmpOps & ~((1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) |
+                                  (1ull << CLIFFT_OP_EXPAND_T_DAG) | (1ull << CLIFFT_OP_EXPAND_ROT)));
+
+    // Record one permutation/phase opcode of the current rank k instead of
+    // sweeping now (false: fusion is off for this team; sweep as usual).
+    CSVM_INLINE bool defer_mono(uint32_t kind, uint32_t a, uint32_t b, uint32_t m, double pr, double pi) {
+        if (!amp_alt) return false;
+        if (nmono == kMaxMono) flush_mono();
+        if (g.lane == 0) {
+            MonoOp o;
+            o.kind = kind;
+            o.a = a;
+            o.b = b;
+            o.m = m;
+            o.pr = pr;
+            o.pi = pi;
+            store_mono(mops + nmono, o);
+        }
+        ++nmono;
+        return true;
+    }
+    // Apply the deferred run: a single opcode in place with its own sweep,
+    // two or more in one out-of-place pass into the other slice.
+    CSVM_INLINE void flush_mono() {
+        if (nmono == 0) return;
+        team_sync<BT>();  // the list written by lane 0 is visible to every wave
+        if (nmono == 1) {
+            const MonoOp o = load_mono(mops);
+            const uint32_t kind = uni(o.kind);
+            if (kind == MONO_CNOT) sweep_two_axis<BT>(amp, k, o.a, o.b, PERM_CNOT, g);
+            else if (kind == MONO_CZ) sweep_two_axis<BT>(amp, k, o.a, o.b, PERM_CZ, g);
+            else if (kind == MONO_SWAP) sweep_two_axis<BT>(amp, k, o.a, o.b, PERM_SWAP, g);
+            else if (kind == MONO_MULTI_CNOT) sweep_multi<BT>(amp, k, o.a, o.m, 0u, g);
+            else if (kind == MONO_MULTI_CZ) sweep_multi<BT>(amp, k, o.a, o.m, 1u, g);
+            else sweep_phase<BT>(amp, k, o.a, o.pr, o.pi, g);
+        } else {
+            sweep_mono<BT>(amp, amp_alt, k, mops, nmono, g);
+            AmpT* t = amp;
+            amp = amp_alt;
+            amp_alt = t;
+        }
+        nmono = 0;
+    }
+
     // svm.h:177-186
     CSVM_INLINE void scale_magnitude(double s) {
         g_re *= s;
         g_im *= s;
         double gm = hypot(g_re, g_im);
         if (gm > 1e100 || (gm < 1e-100 && gm > 0.0)) {
+            flush_mono();
             flush_expand();
             sweep_scale<BT>(amp, k, gm, g);
             g_re /= gm;
@@ -516,6 +574,7 @@ struct Interp {
             const uint32_t a2 = (uint32_t)(w0 >> 48);
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
+            if (nmono && !((kMonoKeepOps >> op) & 1ull)) flush_mono();
             if (npend && !((kNoAmpOps >> op) & 1ull)) flush_expand();
             switch (op) {
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
@@ -545,22 +604,22 @@ struct Interp {
                         else
                             sweep_fanout<BT>(amp, k, a1, tmask, g);
                     } else {
-                        sweep_two_axis<BT>(amp, k, a1, a2, PERM_CNOT, g);
+                        if (!defer_mono(MONO_CNOT, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_CNOT, g);
                         frame_cnot(a1, a2);
                     }
                     break;
                 }
                 case CLIFFT_OP_ARRAY_CZ:  // :255-365
-                    sweep_two_axis<BT>(amp, k, a1, a2, PERM_CZ, g);
+                    if (!defer_mono(MONO_CZ, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_CZ, g);
                     frame_cz(a1, a2);
                     break;
                 case CLIFFT_OP_ARRAY_SWAP:  // :369-458
-                    sweep_two_axis<BT>(amp, k, a1, a2, PERM_SWAP, g);
+                    if (!defer_mono(MONO_SWAP, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_SWAP, g);
                     frame_swap(a1, a2);
                     break;
                 case CLIFFT_OP_ARRAY_MULTI_CNOT: {  // :464-550
                     uint64_t m = w1 & ((1ULL << k) - 1ULL);
-                    sweep_multi<BT>(amp, k, a1, (uint32_t)m, 0u, g);
+                    if (!defer_mono(MONO_MULTI_CNOT, a1, 0u, (uint32_t)m, 0.0, 0.0)) sweep_multi<BT>(amp, k, a1, (uint32_t)m, 0u, g);
                     while (m) {
                         uint32_t c = (uint32_t)__builtin_ctzll(m);
                         m &= m - 1;
@@ -570,7 +629,7 @@ struct Interp {
                 }
                 case CLIFFT_OP_ARRAY_MULTI_CZ: {  // :559-635
                     uint64_t m = w1 & ((1ULL << k) - 1ULL);
-                    sweep_multi<BT>(amp, k, a1, (uint32_t)m, 1u, g);
+                    if (!defer_mono(MONO_MULTI_CZ, a1, 0u, (uint32_t)m, 0.0, 0.0)) sweep_multi<BT>(amp, k, a1, (uint32_t)m, 1u, g);
                     while (m) {
                         uint32_t t = (uint32_t)__builtin_ctzll(m);
                         m &= m - 1;
@@ -583,11 +642,11 @@ struct Interp {
                     frame_h(a1);
                     break;
                 case CLIFFT_OP_ARRAY_S:  // :854-858
-                    sweep_phase<BT>(amp, k, a1, 0.0, 1.0, g);
+                    if (!defer_mono(MONO_PHASE, a1, 0u, 0u, 0.0, 1.0)) sweep_phase<BT>(amp, k, a1, 0.0, 1.0, g);
                     frame_s(a1, 1.0);
                     break;
                 case CLIFFT_OP_ARRAY_S_DAG:  // :862-866
-                    sweep_phase<BT>(amp, k, a1, 0.0, -1.0, g);
+                    if (!defer_mono(MONO_PHASE, a1, 0u, 0u, 0.0, -1.0)) sweep_phase<BT>(amp, k, a1, 0.0, -1.0, g);
                     frame_s(a1, -1.0);
                     break;
                 case CLIFFT_OP_ARRAY_T:
@@ -599,7 +658,8 @@ struct Interp {
                         zi = __longlong_as_double((long long)code[4 * pc + 2]);
                     }
                     uint32_t px = f.gx(a1);
-                    if (a1 < k) sweep_phase<BT>(amp, k, a1, zr, px ? -zi : zi, g);
+                    if (a1 < k && !defer_mono(MONO_PHASE, a1, 0u, 0u, zr, px ? -zi : zi))
+                        sweep_phase<BT>(amp, k, a1, zr, px ? -zi : zi, g);
                     if (px) mul_phase(zr, zi);
                     break;
                 }
@@ -757,6 +817,7 @@ struct Interp {
                 default: break;
             }
         }
+        flush_mono();
         flush_expand();
     }
 };

@@ -80,6 +80,15 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
         S.amp = (AmpT*)(gamp);
     else
         S.amp = (AmpT*)((dv2*)lds_amp + (size_t)team_in_block * amp_elems);
+    // fused permutation/phase runs: workgroup teams on global amplitudes with
+    // a second slice; the opcode list follows the wavefronts' record blocks
+    S.amp_alt = nullptr;
+    S.mops = nullptr;
+    S.nmono = 0;
+    if (GA && BT && A->amp_slices == 2) {
+        S.amp_alt = (AmpT*)(gamp + amp_elems);
+        S.mops = (lds_mono*)(p + (size_t)waves_block * rec_stride);
+    }
 
     for (uint64_t s = s_first; s < s_end; s += s_step) {
         A = opaque(Ag);
@@ -329,7 +338,7 @@ __global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram*
     const uint32_t team_in_block = BT ? 0u : uni(threadIdx.x >> 6);
     const uint64_t team_global = (uint64_t)blockIdx.x * tpb + team_in_block;
     const uint64_t nteams = (uint64_t)gridDim.x * tpb;
-    dv2* gamp = GA ? Ag->gscratch + team_global * Ag->amp_elems : nullptr;
+    dv2* gamp = GA ? Ag->gscratch + team_global * Ag->amp_elems * Ag->amp_slices : nullptr;
     shot_range<FK, BT, GA>(cp(P), Ag, team_global, Ag->shot_count, nteams, gamp, Ag->seed);
 }
 

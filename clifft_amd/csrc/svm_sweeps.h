@@ -295,6 +295,81 @@ CSVM_SWEEP void sweep_u4(AmpT* amp, uint32_t k, uint32_t lo, uint32_t hi, cptr<d
     team_sync<BT>();
 }
 
+// A run of permutation/phase opcodes in one out-of-place pass: dst[y] is the
+// entry that the run carries to index y.  Its source x is found by undoing
+// the permutations (each is its own inverse) in reverse order; the entry is
+// then walked forward through the run, multiplied by each phase at the index
+// it holds at that point, exactly as the one-at-a-time sweeps would (same
+// products in the same order, so the values are bit-identical).  Each index
+// stays below 2^k: the opcodes only exchange or flip bits below k.
+typedef __attribute__((address_space(3))) MonoOp lds_mono;
+CSVM_INLINE MonoOp load_mono(const lds_mono* p) {
+    MonoOp o;
+    o.kind = p->kind;
+    o.a = p->a;
+    o.b = p->b;
+    o.m = p->m;
+    o.pr = p->pr;
+    o.pi = p->pi;
+    return o;
+}
+CSVM_INLINE void store_mono(lds_mono* p, const MonoOp& o) {
+    p->kind = o.kind;
+    p->a = o.a;
+    p->b = o.b;
+    p->m = o.m;
+    p->pr = o.pr;
+    p->pi = o.pi;
+}
+CSVM_INLINE uint32_t mono_perm(const MonoOp& o, uint32_t x) {
+    if (o.kind == MONO_CNOT) return x ^ (((x >> o.a) & 1u) << o.b);
+    if (o.kind == MONO_SWAP) {
+        uint32_t d = ((x >> o.a) ^ (x >> o.b)) & 1u;
+        return x ^ ((d << o.a) | (d << o.b));
+    }
+    if (o.kind == MONO_MULTI_CNOT) return x ^ (((uint32_t)__popc(x & o.m) & 1u) << o.a);
+    return x;
+}
+CSVM_INLINE dv2 mono_phase(const MonoOp& o, uint32_t x, dv2 v) {
+    if (o.kind == MONO_CZ) return ((x >> o.a) & (x >> o.b) & 1u) ? cneg(v) : v;
+    if (o.kind == MONO_MULTI_CZ) return (((x >> o.a) & 1u) && (__popc(x & o.m) & 1u)) ? cneg(v) : v;
+    if (o.kind == MONO_PHASE) return ((x >> o.a) & 1u) ? cmul(v, mk(o.pr, o.pi)) : v;
+    return v;
+}
+template <bool BT, class AmpT>
+CSVM_SWEEP void sweep_mono(const AmpT* src, AmpT* dst, uint32_t k, const lds_mono* ops, uint32_t n, TeamGeom g) {
+    k = uni(k); n = uni(n); g = uni_geom(g);
+    const uint32_t K = 1u << k;
+    constexpr uint32_t U = Unroll<AmpT>::value;
+    const uint32_t step = U * g.size;
+    for (uint32_t y0 = g.lane; y0 < K; y0 += step) {
+        uint32_t x[U];
+        dv2 v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) x[u] = y0 + u * g.size;
+        for (uint32_t j = n; j-- > 0;) {
+            const MonoOp o = load_mono(ops + j);
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) x[u] = mono_perm(o, x[u]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (y0 + u * g.size < K) v[u] = src[x[u]];
+        for (uint32_t j = 0; j < n; ++j) {
+            const MonoOp o = load_mono(ops + j);
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                v[u] = mono_phase(o, x[u], v[u]);
+                x[u] = mono_perm(o, x[u]);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (y0 + u * g.size < K) dst[y0 + u * g.size] = v[u];
+    }
+    team_sync<BT>();
+}
+
 // EXPAND: amp[i + 2^k] = amp[i] * ph for i < 2^k (:876-945)
 template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_expand(AmpT* amp, uint32_t k, double pr, double pi, uint32_t has_phase, TeamGeom g) {
