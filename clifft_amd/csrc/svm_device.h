@@ -139,6 +139,27 @@ struct ShotArgs {
     uint32_t pad0;
 };
 
+// Opcodes that never read or write amplitudes nor depend on them, so
+// pending expansions may stay pending across them.
+constexpr uint64_t kNoAmpOps =
+    (1ull << CLIFFT_OP_FRAME_CNOT) | (1ull << CLIFFT_OP_FRAME_CZ) | (1ull << CLIFFT_OP_FRAME_H) |
+    (1ull << CLIFFT_OP_FRAME_S) | (1ull << CLIFFT_OP_FRAME_S_DAG) | (1ull << CLIFFT_OP_FRAME_SWAP) |
+    (1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) | (1ull << CLIFFT_OP_EXPAND_T_DAG) |
+    (1ull << CLIFFT_OP_EXPAND_ROT) | (1ull << CLIFFT_OP_MEAS_DORMANT_STATIC) |
+    (1ull << CLIFFT_OP_MEAS_DORMANT_RANDOM) | (1ull << CLIFFT_OP_APPLY_PAULI) | (1ull << CLIFFT_OP_NOISE) |
+    (1ull << CLIFFT_OP_NOISE_BLOCK) | (1ull << CLIFFT_OP_READOUT_NOISE) | (1ull << CLIFFT_OP_DETECTOR) |
+    (1ull << CLIFFT_OP_OBSERVABLE);
+// Permutation/phase opcodes that a fused run may hold, and what may sit
+// between them (the amplitude-free opcodes other than EXPAND*).
+constexpr uint64_t kMonoOps =
+    (1ull << CLIFFT_OP_ARRAY_CNOT) | (1ull << CLIFFT_OP_ARRAY_CZ) | (1ull << CLIFFT_OP_ARRAY_SWAP) |
+    (1ull << CLIFFT_OP_ARRAY_MULTI_CNOT) | (1ull << CLIFFT_OP_ARRAY_MULTI_CZ) | (1ull << CLIFFT_OP_ARRAY_S) |
+    (1ull << CLIFFT_OP_ARRAY_S_DAG) | (1ull << CLIFFT_OP_ARRAY_T) | (1ull << CLIFFT_OP_ARRAY_T_DAG) |
+    (1ull << CLIFFT_OP_ARRAY_ROT);
+constexpr uint64_t kMonoKeepOps =
+    kMonoOps | (kNoAmpOps & ~((1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) |
+                              (1ull << CLIFFT_OP_EXPAND_T_DAG) | (1ull << CLIFFT_OP_EXPAND_ROT)));
+
 // One deferred permutation/phase opcode of a fused run (global tier): the
 // run's opcodes are applied to every amplitude in one out-of-place pass.
 enum : uint32_t { MONO_CNOT = 0, MONO_CZ = 1, MONO_SWAP = 2, MONO_MULTI_CNOT = 3, MONO_MULTI_CZ = 4, MONO_PHASE = 5 };

@@ -289,6 +289,7 @@ struct clifft_svm_program {
     uint32_t num_meas = 0, total_slots = 0, num_det = 0, num_obs = 0, num_ev = 0;
     bool has_postselection = false;
     bool has_forced_ops = false;  // bytecode already contains *_FORCED opcodes
+    bool has_mono_runs = false;   // some run of >= 2 permutation/phase opcodes could be fused
     uint64_t num_array_ops = 0;
     double bytes_per_shot = 0.0;
     // importance sampling: per-site fault probabilities (quantum sites, then
@@ -400,7 +401,10 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
     // Workgroup teams on global amplitudes get a second slice and an LDS
     // list for fused permutation/phase runs (one read and one write of the
     // state per run instead of one pass per opcode).
-    if (L.global_amps && L.block_team && fuse_mono_enabled() && lds + kMonoBytes <= 160 * 1024) {
+    // Only programs with a run of two or more, and states of at most 2^24
+    // amplitudes (a second 512 MiB slice per team at most).
+    if (L.global_amps && L.block_team && fuse_mono_enabled() && p->has_mono_runs && k <= 24 &&
+        lds + kMonoBytes <= 160 * 1024) {
         L.amp_slices = 2;
         lds += kMonoBytes;
     }
@@ -553,11 +557,22 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     if (view->expected_observables)
         p->expected_obs.assign(view->expected_observables, view->expected_observables + view->num_expected_observables);
     const uint32_t fbits = view->words * 64;
+    uint32_t run = 0;
     for (uint64_t i = 0; i < view->num_instructions; ++i) {
-        double b = op_bytes(p->host_bc[i], khist[i], fbits);
+        const clifft_instruction& in = p->host_bc[i];
+        double b = op_bytes(in, khist[i], fbits);
         if (b > 0) {
             p->num_array_ops++;
             p->bytes_per_shot += b;
+        }
+        // the interpreter's deferral rule: T/ROT on a dormant axis sweep nothing
+        const uint64_t bit = in.opcode < 64 ? 1ull << in.opcode : 0ull;
+        const bool dormant_phase = (in.opcode == CLIFFT_OP_ARRAY_T || in.opcode == CLIFFT_OP_ARRAY_T_DAG ||
+                                    in.opcode == CLIFFT_OP_ARRAY_ROT) && in.axis_1 >= (uint32_t)khist[i];
+        if ((kMonoOps & bit) && !dormant_phase) {
+            if (++run >= 2) p->has_mono_runs = true;
+        } else if (!(kMonoKeepOps & bit)) {
+            run = 0;
         }
     }
 

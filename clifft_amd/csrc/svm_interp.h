@@ -134,27 +134,7 @@ struct Interp {
         npend = 0;
         pend_mask = 0;
     }
-    // Opcodes that never read or write amplitudes nor depend on them, so
-    // pending expansions may stay pending across them.
-    static constexpr uint64_t kNoAmpOps =
-        (1ull << CLIFFT_OP_FRAME_CNOT) | (1ull << CLIFFT_OP_FRAME_CZ) | (1ull << CLIFFT_OP_FRAME_H) |
-        (1ull << CLIFFT_OP_FRAME_S) | (1ull << CLIFFT_OP_FRAME_S_DAG) | (1ull << CLIFFT_OP_FRAME_SWAP) |
-        (1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) | (1ull << CLIFFT_OP_EXPAND_T_DAG) |
-        (1ull << CLIFFT_OP_EXPAND_ROT) | (1ull << CLIFFT_OP_MEAS_DORMANT_STATIC) |
-        (1ull << CLIFFT_OP_MEAS_DORMANT_RANDOM) | (1ull << CLIFFT_OP_APPLY_PAULI) | (1ull << CLIFFT_OP_NOISE) |
-        (1ull << CLIFFT_OP_NOISE_BLOCK) | (1ull << CLIFFT_OP_READOUT_NOISE) | (1ull << CLIFFT_OP_DETECTOR) |
-        (1ull << CLIFFT_OP_OBSERVABLE);
 
-    // Permutation/phase opcodes that a fused run may hold: runs of them
-    // (with kNoAmpOps other than EXPAND* in between) become one pass.
-    static constexpr uint64_t kMonoOps =
-        (1ull << CLIFFT_OP_ARRAY_CNOT) | (1ull << CLIFFT_OP_ARRAY_CZ) | (1ull << CLIFFT_OP_ARRAY_SWAP) |
-        (1ull << CLIFFT_OP_ARRAY_MULTI_CNOT) | (1ull << CLIFFT_OP_ARRAY_MULTI_CZ) | (1ull << CLIFFT_OP_ARRAY_S) |
-        (1ull << CLIFFT_OP_ARRAY_S_DAG) | (1ull << CLIFFT_OP_ARRAY_T) | (1ull << CLIFFT_OP_ARRAY_T_DAG) |
-        (1ull << CLIFFT_OP_ARRAY_ROT);
-    static constexpr uint64_t kMonoKeepOps =
-        kMonoOps | (kNoAmpOps & ~((1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) |
-                                  (1ull << CLIFFT_OP_EXPAND_T_DAG) | (1ull << CLIFFT_OP_EXPAND_ROT)));
 
     // Record one permutation/phase opcode of the current rank k instead of
     // sweeping now (false: fusion is off for this team; sweep as usual).
