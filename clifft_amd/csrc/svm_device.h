@@ -37,12 +37,6 @@ constexpr double kInvSqrt2 = 0.70710678118654752440;
 constexpr double kDustEpsilon = 1e-18;
 constexpr double kUnreachable = -1.7976931348623157e308;
 constexpr uint32_t kNoNoise = 0xFFFFFFFFu;
-// Fusion of consecutive same-control ARRAY_CNOTs into one fan-out sweep.
-// Switched off: the first device run that carried it ended in a GPU memory
-// fault whose cause was not found (a CPU emulation of these kernels under
-// AddressSanitizer found no out-of-bounds access), so the kernels stay the
-// ones that last passed every device test.
-constexpr bool kFuseFanoutCnot = false;
 
 // Complex amplitude: a native two-lane double vector (loads/stores are single
 // 16-byte memory operations in any address space).

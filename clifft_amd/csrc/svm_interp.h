@@ -563,32 +563,10 @@ struct Interp {
                 case CLIFFT_OP_FRAME_S: frame_s(a1, 1.0); break;
                 case CLIFFT_OP_FRAME_S_DAG: frame_s(a1, -1.0); break;
                 case CLIFFT_OP_FRAME_SWAP: frame_swap(a1, a2); break;
-                case CLIFFT_OP_ARRAY_CNOT: {  // :124-252
-                    if constexpr (kFuseFanoutCnot) {
-                        // the CNOTs that follow with the same control and new
-                        // targets join one fan-out sweep (a permutation, so
-                        // the values are those of the one-at-a-time sweeps)
-                        uint32_t tmask = 1u << a2;
-                        frame_cnot(a1, a2);
-                        while (pc + 1 < n_instr) {
-                            const uint64_t nx = code[4 * (pc + 1)];
-                            if ((uint32_t)(nx & 0xFF) != CLIFFT_OP_ARRAY_CNOT || (uint32_t)((nx >> 32) & 0xFFFF) != a1) break;
-                            const uint32_t t = (uint32_t)(nx >> 48);
-                            if (tmask & (1u << t)) break;
-                            tmask |= 1u << t;
-                            frame_cnot(a1, t);
-                            ++pc;
-                        }
-                        if (tmask == (1u << a2))
-                            sweep_two_axis<BT>(amp, k, a1, a2, PERM_CNOT, g);
-                        else
-                            sweep_fanout<BT>(amp, k, a1, tmask, g);
-                    } else {
-                        if (!defer_mono(MONO_CNOT, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_CNOT, g);
-                        frame_cnot(a1, a2);
-                    }
+                case CLIFFT_OP_ARRAY_CNOT:  // :124-252
+                    if (!defer_mono(MONO_CNOT, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_CNOT, g);
+                    frame_cnot(a1, a2);
                     break;
-                }
                 case CLIFFT_OP_ARRAY_CZ:  // :255-365
                     if (!defer_mono(MONO_CZ, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_CZ, g);
                     frame_cz(a1, a2);

@@ -349,40 +349,15 @@ struct LaneInterp {
                 case CLIFFT_OP_FRAME_S_DAG: frame_s(a1, -1.0); break;
                 case CLIFFT_OP_FRAME_SWAP: frame_swap(a1, a2); break;
                 case CLIFFT_OP_ARRAY_CNOT: {
-                    if constexpr (kFuseFanoutCnot) {
-                        // CNOTs that follow with the same control and new
-                        // targets join one fan-out pass (a permutation)
-                        uint32_t tmask = 1u << a2;
-                        frame_cnot(a1, a2);
-                        while (pc + 1 < n_instr) {
-                            const uint64_t nx = code[4 * (pc + 1)];
-                            if ((uint32_t)(nx & 0xFF) != CLIFFT_OP_ARRAY_CNOT || (uint32_t)((nx >> 32) & 0xFFFF) != a1) break;
-                            const uint32_t t = (uint32_t)(nx >> 48);
-                            if (tmask & (1u << t)) break;
-                            tmask |= 1u << t;
-                            frame_cnot(a1, t);
-                            ++pc;
-                        }
-                        const uint32_t t0 = (uint32_t)__builtin_ctz(tmask);
-                        const uint32_t n = 1u << (ku() - 2), cb = 1u << a1;
-                        const uint32_t lo = a1 < t0 ? a1 : t0, hi = a1 < t0 ? t0 : a1;
-                        for (uint32_t i = 0; i < n; ++i) {
-                            uint32_t b0 = insert_two_zero_bits(i, lo, hi) | cb, b1 = b0 ^ tmask;
-                            dv2 x = A(b0), y = A(b1);
-                            S(b0, y);
-                            S(b1, x);
-                        }
-                    } else {
-                        const uint32_t n = 1u << (ku() - 2), ab = 1u << a1, bb = 1u << a2;
-                        const uint32_t lo = a1 < a2 ? a1 : a2, hi = a1 < a2 ? a2 : a1;
-                        for (uint32_t i = 0; i < n; ++i) {
-                            uint32_t b0 = insert_two_zero_bits(i, lo, hi) | ab, b1 = b0 | bb;
-                            dv2 x = A(b0), y = A(b1);
-                            S(b0, y);
-                            S(b1, x);
-                        }
-                        frame_cnot(a1, a2);
+                    const uint32_t n = 1u << (ku() - 2), ab = 1u << a1, bb = 1u << a2;
+                    const uint32_t lo = a1 < a2 ? a1 : a2, hi = a1 < a2 ? a2 : a1;
+                    for (uint32_t i = 0; i < n; ++i) {
+                        uint32_t b0 = insert_two_zero_bits(i, lo, hi) | ab, b1 = b0 | bb;
+                        dv2 x = A(b0), y = A(b1);
+                        S(b0, y);
+                        S(b1, x);
                     }
+                    frame_cnot(a1, a2);
                     break;
                 }
                 case CLIFFT_OP_ARRAY_CZ: {

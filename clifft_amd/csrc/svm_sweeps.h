@@ -119,40 +119,6 @@ CSVM_SWEEP void sweep_two_axis(AmpT* amp, uint32_t k, uint32_t a, uint32_t b, ui
     team_sync<BT>();
 }
 
-// Consecutive CNOTs sharing the control c (targets tmask, c not in tmask):
-// x -> x ^ (x_c ? tmask : 0), one pass over the control-set half.  Pairs
-// are visited once, from the member whose lowest target bit is clear.
-template <bool BT, class AmpT>
-CSVM_SWEEP void sweep_fanout(AmpT* amp, uint32_t k, uint32_t c, uint32_t tmask, TeamGeom g) {
-    k = uni(k); c = uni(c); tmask = uni(tmask); g = uni_geom(g);
-    const uint32_t t0 = (uint32_t)__builtin_ctz(tmask);
-    const uint32_t lo = c < t0 ? c : t0, hi = c < t0 ? t0 : c, cb = 1u << c;
-    const uint32_t n = 1u << (k - 2);
-    constexpr uint32_t U = Unroll<AmpT>::value;
-    const uint32_t step = U * g.size;
-    for (uint32_t j0 = g.lane; j0 < n; j0 += step) {
-        dv2 x[U], y[U];
-        uint32_t ia[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t j = j0 + u * g.size;
-            ia[u] = insert_two_zero_bits(j, lo, hi) | cb;
-            if (j < n) {
-                x[u] = amp[ia[u]];
-                y[u] = amp[ia[u] ^ tmask];
-            }
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            if (j0 + u * g.size < n) {
-                amp[ia[u]] = y[u];
-                amp[ia[u] ^ tmask] = x[u];
-            }
-        }
-    }
-    team_sync<BT>();
-}
-
 // MULTI_CNOT (is_cz = 0): swap the target pair where parity(idx & mask) is
 // odd.  MULTI_CZ (is_cz = 1): negate entries with the control set and odd
 // parity(idx & mask).  `mask` is already restricted to active axes.

@@ -242,7 +242,7 @@ def random_measure_only_program(seed, num_qubits=8, peak=5, length=120):
 
 def fanout_program(seed, peak, num_qubits=None, frames=True):
     """Expansions to rank `peak`, then runs of ARRAY_CNOTs sharing a control
-    (the device fuses each run into one fan-out sweep), mixed with H/T, noise
+    (in the global tier the device applies each run in one fused pass), mixed with H/T, noise
     on the frame, EXP_VAL probes and final measurements of every axis."""
     from clifft_amd.program import (Op, build_program, make_array_cnot, make_array_h, make_array_t,
                                     make_exp_val, make_expand_t, make_frame_h, make_meas, make_noise)

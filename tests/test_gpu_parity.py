@@ -136,9 +136,9 @@ def test_reference_stream_first_shot(svm, orc, peak):
 
 @pytest.mark.parametrize("peak", [3, 5, 9, 15])
 def test_fanout_cnot_runs(svm, orc, peak):
-    """Runs of CNOTs sharing a control (the pattern the fan-out fusion,
-    compiled out while kFuseFanoutCnot is false, would merge), lane and team
-    tiers, against the oracle's one-CNOT-at-a-time execution."""
+    """Runs of CNOTs sharing a control, lane and team tiers (peak 15: the
+    global tier applies each run in one fused pass), against the oracle's
+    one-CNOT-at-a-time execution."""
     from randprog import fanout_program
     announce("fanout", peak)
     for seed in range(3):
