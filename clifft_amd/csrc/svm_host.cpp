@@ -353,6 +353,17 @@ bool fuse_mono_enabled() {
     return v;
 }
 
+// Threads of a workgroup team on global amplitudes (CLIFFT_GLOBAL_TEAM_THREADS:
+// 256, 512 or 1024).
+uint32_t global_team_threads() {
+    static uint32_t v = [] {
+        const char* e = std::getenv("CLIFFT_GLOBAL_TEAM_THREADS");
+        const int t = e ? std::atoi(e) : 1024;
+        return (uint32_t)(t == 256 || t == 512 ? t : 1024);
+    }();
+    return v;
+}
+
 // Choose the team shape and storage tier for a program's peak rank.
 LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool kfault = false) {
     LaunchShape L;
@@ -388,7 +399,7 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
         L.global_amps = false;
     } else {
         L.block_team = true;
-        L.block = 1024;
+        L.block = global_team_threads();
         L.global_amps = true;
     }
     const uint32_t waves = L.block / 64;
