@@ -338,7 +338,7 @@ int lane_max_rank() {
 size_t global_workset_bytes() {
     static size_t v = [] {
         const char* e = std::getenv("CLIFFT_GLOBAL_WORKSET_MIB");
-        return (size_t)(e ? std::atoll(e) : 256) << 20;
+        return (size_t)(e ? std::atoll(e) : 0) << 20;
     }();
     return v;
 }
@@ -353,13 +353,14 @@ bool fuse_mono_enabled() {
     return v;
 }
 
-// Threads of a workgroup team on global amplitudes (CLIFFT_GLOBAL_TEAM_THREADS:
-// 256, 512 or 1024).
+// Threads of a workgroup team on global amplitudes: CLIFFT_GLOBAL_TEAM_THREADS
+// (256, 512 or 1024) fixes it; 0 (unset) lets finalize_shape pick it from
+// the shot count.
 uint32_t global_team_threads() {
     static uint32_t v = [] {
         const char* e = std::getenv("CLIFFT_GLOBAL_TEAM_THREADS");
-        const int t = e ? std::atoi(e) : 1024;
-        return (uint32_t)(t == 256 || t == 512 ? t : 1024);
+        const int t = e ? std::atoi(e) : 0;
+        return (uint32_t)(t == 256 || t == 512 || t == 1024 ? t : 0);
     }();
     return v;
 }
@@ -399,10 +400,11 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
         L.global_amps = false;
     } else {
         L.block_team = true;
-        L.block = global_team_threads();
+        L.block = global_team_threads() ? global_team_threads() : 1024;
         L.global_amps = true;
     }
     const uint32_t waves = L.block / 64;
+    L.rec_stride = (uint32_t)rec_stride;
     size_t lds = red + waves * rec_stride;
     if (!L.global_amps) lds += L.teams_per_block() * amp_bytes;
     if (lds > 160 * 1024) {  // records too large next to LDS amplitudes
@@ -456,8 +458,35 @@ int ensure_out(clifft_svm_program* p, size_t bytes) {
 }
 
 // Size the grid: persistent teams up to the resident capacity of the chip.
+// Workgroup teams on global amplitudes: the smallest team (256, 512, 1024
+// threads) whose resident teams the launch still fills to 3/4.  Smaller
+// teams keep more states in flight; measured on one MI355X (random
+// Clifford+T, per-shot time): peak 14 and 15 at 256 threads 3.0x and 2.3x
+// faster than at 1024, peak 16 1.5x, peak 18 best at 512 with 500 shots,
+// peak 20 and 22 (125 and 64 shots) best at 1024.
+int choose_global_team(LaunchShape& L, uint64_t shots) {
+    if (!L.global_amps || !L.block_team || global_team_threads()) return CLIFFT_OK;
+    DeviceInfo& di = device_info();
+    for (uint32_t t : {256u, 512u, 1024u}) {
+        LaunchShape c = L;
+        c.block = t;
+        c.lds_bytes = kTeamScratchBytes + (t / 64) * (size_t)L.rec_stride + (L.amp_slices == 2 ? kMonoBytes : 0);
+        int per_cu = 0;
+        hipError_t e = shot_occupancy(c, &per_cu);
+        if (e != hipSuccess) return fail(CLIFFT_E_HIP, std::string("occupancy query: ") + hipGetErrorString(e));
+        const uint64_t slots = (uint64_t)di.num_cus * (uint64_t)std::max(per_cu, 1);
+        if (t == 1024u || 4 * shots >= 3 * slots) {
+            L = c;
+            return CLIFFT_OK;
+        }
+    }
+    return CLIFFT_OK;
+}
+
 int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStream_t st, void** gscratch) {
     DeviceInfo& di = device_info();
+    int trc = choose_global_team(L, shots);
+    if (trc) return trc;
     const uint32_t tpb = L.teams_per_block();
     uint64_t blocks_needed = (shots + tpb - 1) / tpb;
     int per_cu_i = 0;
@@ -1713,6 +1742,11 @@ int clifft_svm_adjoint_create(clifft_svm_program* p, const uint32_t* param_instr
     const size_t red_dens = 16 * 4 * sizeof(double) + align_up((size_t)std::max<uint32_t>(p->num_ev, 1) * 8, 16);
     a->global_amps = k > 12 || red_dens + ((size_t)32 << k) > 160 * 1024;
     a->block = k <= 6 ? 64 : (k <= 8 ? 256 : (k <= 10 ? 512 : 1024));
+    if (a->global_amps) {  // CLIFFT_ADJOINT_TEAM_THREADS: 256, 512 or 1024
+        const char* e = std::getenv("CLIFFT_ADJOINT_TEAM_THREADS");
+        const int t = e ? std::atoi(e) : 0;
+        if (t == 256 || t == 512 || t == 1024) a->block = (uint32_t)t;
+    }
     a->lds_bytes = red_dens + (a->global_amps ? 0 : ((size_t)32 << k));
     if (a->lds_bytes > 160 * 1024) {
         clifft_svm_adjoint_destroy(a);
