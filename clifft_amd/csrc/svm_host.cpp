@@ -506,9 +506,11 @@ int finalize_shape(clifft_svm_program* p, LaunchShape& L, uint64_t shots, hipStr
         size_t per_team = ((size_t)16 << p->peak_rank) * L.amp_slices;
         uint64_t max_teams = std::max<uint64_t>(1, budget / per_team);
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, max_teams / tpb));
-        // Keep the states in flight within the last-level cache budget
-        // (CLIFFT_GLOBAL_WORKSET_MIB, default 256 MiB; 0 = no cap) so each
-        // shot's repeated sweeps hit cache, but never below one team per CU.
+        // Optionally keep the states in flight within a cache budget
+        // (CLIFFT_GLOBAL_WORKSET_MIB; default 0 = no cap), never below one
+        // team per CU.  Measured on one MI355X, a 256 MiB cap made no
+        // difference with 1024-thread teams and cost 2.2x at peak 16 with
+        // 256-thread teams, so it is off.
         const size_t ws = global_workset_bytes();
         if (ws > 0) {
             const uint64_t by_ws = std::max<uint64_t>((uint64_t)di.num_cus, ws / per_team);
