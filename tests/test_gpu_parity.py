@@ -148,3 +148,19 @@ def test_fanout_cnot_runs(svm, orc, peak):
         m, d, o, e, _ = orc.sample(p, shots, seed=seed, per_shot=True)
         assert np.array_equal(res.measurements, m)
         np.testing.assert_allclose(res.exp_vals, e, atol=ATOL, rtol=0)
+
+
+@pytest.mark.parametrize("peak,shots,seed", [(14, 1100, 4263), (15, 600, 4257)])
+def test_global_tier_team_sizes(svm, orc, peak, shots, seed):
+    """Enough shots that the launch picks smaller global-memory teams (256
+    threads at peak 14, 512 at peak 15); the programs reach their peak rank
+    and hold 23 and 27 permutation/phase opcodes at rank >= 10 (fused runs):
+    records bit-exact and expectation values within 1e-12 of the oracle."""
+    announce("global-teams", peak, shots, seed)
+    p = random_program(seed, num_qubits=peak + 6, peak=peak, length=220, p_array=0.5)
+    r = svm.sample(p, shots, seed=99)
+    m, d, o, e, _ = orc.sample(p, shots, seed=99, per_shot=True)
+    assert np.array_equal(r.measurements, m)
+    assert np.array_equal(r.detectors, d)
+    assert np.array_equal(r.observables, o)
+    np.testing.assert_allclose(r.exp_vals, e, atol=ATOL, rtol=0)
