@@ -1,4 +1,6 @@
 // svm_kernels.hip -- team-interpreter kernels for gfx950 and their launchers.
+#include <atomic>
+
 #include "svm_internal.h"
 #include "svm_interp.h"
 #include "svm_lane.h"
@@ -464,12 +466,12 @@ __global__ void clear_u32_kernel(uint32_t* p, uint32_t n) {
 template <int FK, bool BT, bool GA>
 static hipError_t launch_one(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
     auto kern = shot_kernel<FK, BT, GA>;
-    static bool attr_done = false;
-    if (!attr_done) {
+    static std::atomic<bool> attr_done{false};
+    if (!attr_done.load(std::memory_order_acquire)) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
-        attr_done = true;
+        attr_done.store(true, std::memory_order_release);
     }
     hipLaunchKernelGGL(kern, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
     return hipGetLastError();
@@ -480,11 +482,11 @@ static const void* lane_fn(const LaunchShape& L) {
 }
 
 static hipError_t launch_lane(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
-    static bool attr_done[2] = {false, false};
-    if (!attr_done[L.lane_dense]) {
+    static std::atomic<bool> attr_done[2] = {{false}, {false}};
+    if (!attr_done[L.lane_dense].load(std::memory_order_acquire)) {
         hipError_t e = hipFuncSetAttribute(lane_fn(L), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
-        attr_done[L.lane_dense] = true;
+        attr_done[L.lane_dense].store(true, std::memory_order_release);
     }
     if (L.lane_dense)
         hipLaunchKernelGGL(lane_kernel<6>, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
@@ -531,12 +533,12 @@ static const void* batch_fn() {
 }
 
 hipError_t batch_prepare(int nw) {
-    static bool done[2] = {false, false};
+    static std::atomic<bool> done[2] = {{false}, {false}};
     const void* fn = nw == 1 ? batch_fn<FRAME_W1>() : batch_fn<FRAME_LANES>();
-    if (done[nw == 1]) return hipSuccess;
+    if (done[nw == 1].load(std::memory_order_acquire)) return hipSuccess;
     // the queue head is 16 B of static LDS; dynamic LDS may use the rest
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kBatchMaxDynLds);
-    if (e == hipSuccess) done[nw == 1] = true;
+    if (e == hipSuccess) done[nw == 1].store(true, std::memory_order_release);
     return e;
 }
 
