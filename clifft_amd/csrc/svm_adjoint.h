@@ -122,12 +122,107 @@ CSVM_SWEEP void sweep_zero(AmpT* amp, uint32_t n, TeamGeom g) {
 // --------------------------------------------------------------------------
 // The forward/backward walk of one theta vector by one workgroup team
 // --------------------------------------------------------------------------
+// Consecutive EXP_VAL probes share one pass over phi (forward) and one pass
+// over lambda (backward), at most kProbeGroup at a time.
+typedef __attribute__((address_space(3))) ProbeRec lds_probe;
+CSVM_INLINE void store_probe(lds_probe* p, uint32_t xa, uint32_t za, uint32_t out, double wr, double wi) {
+    p->xa = xa;
+    p->za = za;
+    p->out = out;
+    p->pad = 0;
+    p->wr = wr;
+    p->wi = wi;
+}
+
+CSVM_INLINE ProbeRec load_probe(const lds_probe* p) {
+    ProbeRec r;
+    r.xa = p->xa;
+    r.za = p->za;
+    r.out = p->out;
+    r.pad = 0;
+    r.wr = p->wr;
+    r.wi = p->wi;
+    return r;
+}
+
+// The sums of sweep_exp_val for n probes in one pass: out[2p], out[2p + 1]
+// the numerator of probe p, out[2 kProbeGroup] the shared norm.
+template <class AmpT>
+CSVM_SWEEP void sweep_exp_val_group(const AmpT* amp, uint32_t k, const lds_probe* pr, uint32_t n, TeamGeom g,
+                                    lds_f64* red, double (&out)[2 * kProbeGroup + 4]) {
+    k = uni(k); n = uni(n); g = uni_geom(g);
+    uint32_t xa[kProbeGroup], za[kProbeGroup];
+#pragma unroll
+    for (uint32_t p = 0; p < kProbeGroup; ++p) {
+        xa[p] = p < n ? uni(pr[p].xa) : 0u;
+        za[p] = p < n ? uni(pr[p].za) : 0u;
+    }
+    const uint32_t dim = 1u << k;
+#pragma unroll
+    for (uint32_t i = 0; i < 2 * kProbeGroup + 4; ++i) out[i] = 0.0;
+    for (uint32_t j = g.lane; j < dim; j += g.size) {
+        const dv2 a = amp[j];
+        out[2 * kProbeGroup] += a.x * a.x + a.y * a.y;
+#pragma unroll
+        for (uint32_t p = 0; p < kProbeGroup; ++p) {
+            if (p < n) {
+                const dv2 b = amp[j ^ xa[p]];
+                const double zs = (__popc(j & za[p]) & 1u) ? -1.0 : 1.0;
+                const double cr = zs * a.x, ci = zs * a.y;
+                out[2 * p] += b.x * cr + b.y * ci;
+                out[2 * p + 1] += b.x * ci - b.y * cr;
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 2 * kProbeGroup + 4; c += 4) {
+        double t[4] = {out[c], out[c + 1], out[c + 2], out[c + 3]};
+        team_reduce<true, 4>(t, g, red);
+        out[c] = t[0];
+        out[c + 1] = t[1];
+        out[c + 2] = t[2];
+        out[c + 3] = t[3];
+    }
+}
+
+// lambda[j] += sum_p w_p (-1)^popc((j ^ xa_p) & za_p) phi[j ^ xa_p], the
+// terms added in list order (the order of the one-probe sweeps).
+template <class AmpT>
+CSVM_SWEEP void sweep_pauli_acc_group(AmpT* lam, const AmpT* phi, uint32_t k, const lds_probe* pr, uint32_t n,
+                                      TeamGeom g) {
+    k = uni(k); n = uni(n); g = uni_geom(g);
+    uint32_t xa[kProbeGroup], za[kProbeGroup];
+    dv2 w[kProbeGroup];
+#pragma unroll
+    for (uint32_t p = 0; p < kProbeGroup; ++p) {
+        xa[p] = p < n ? uni(pr[p].xa) : 0u;
+        za[p] = p < n ? uni(pr[p].za) : 0u;
+        w[p] = p < n ? mk(unid(pr[p].wr), unid(pr[p].wi)) : mk(0.0, 0.0);
+    }
+    const uint32_t dim = 1u << k;
+    for (uint32_t j = g.lane; j < dim; j += g.size) {
+        dv2 acc = lam[j];
+#pragma unroll
+        for (uint32_t p = 0; p < kProbeGroup; ++p) {
+            if (p < n) {
+                const uint32_t src = j ^ xa[p];
+                dv2 t = cmul(w[p], phi[src]);
+                if (__popc(src & za[p]) & 1u) t = cneg(t);
+                acc = cadd(acc, t);
+            }
+        }
+        lam[j] = acc;
+    }
+    team_sync<true>();
+}
+
 template <class AmpT>
 struct AdjInterp {
     TeamGeom g;
     AmpT* phi;
     AmpT* lam;
     lds_f64* red;
+    lds_probe* probes;  // kProbeGroup records in LDS
     uint32_t k;
 
     // phase of instruction pc: the theta-vector's parameter or the payload
@@ -239,20 +334,47 @@ struct AdjInterp {
                     break;
                 }
                 case CLIFFT_OP_EXP_VAL: {
-                    uint32_t xa, za;
-                    double ph_r, ph_i;
-                    double res = 0.0, den = 1.0;
-                    if (probe(opaque(Pg), i0, xa, za, ph_r, ph_i)) {
-                        double3 s = sweep_exp_val<true>(phi, k, xa, za, g, red);
-                        double num_re = s.x * ph_r - s.y * ph_i;
-                        den = s.z;
-                        res = den > 0.0 ? num_re / den : 0.0;
-                        if (res > 1.0) res = 1.0;
-                        if (res < -1.0) res = -1.0;
+                    // this probe and the EXP_VALs that follow it (frame
+                    // opcodes between them act on the zero frame only)
+                    cptr<DevProgram> P = opaque(Pg);
+                    uint32_t ng = 0, last = pc;
+                    for (uint32_t t = pc; t < n_instr; ++t) {
+                        const uint64_t u0 = code[4 * t], u1 = code[4 * t + 1];
+                        const uint32_t uop = (uint32_t)(u0 & 0xFF);
+                        if (uop <= CLIFFT_OP_FRAME_SWAP) continue;
+                        if (uop != CLIFFT_OP_EXP_VAL || ng == kProbeGroup) break;
+                        last = t;
+                        const uint32_t ci = (uint32_t)u1, oi = (uint32_t)(u1 >> 32);
+                        uint32_t xa, za;
+                        double ph_r, ph_i;
+                        if (probe(P, ci, xa, za, ph_r, ph_i)) {
+                            if (g.lane == 0) store_probe(probes + ng, xa, za, oi, ph_r, ph_i);
+                            ++ng;
+                        } else if (g.lane == 0) {  // X on a dormant axis: <P> = 0
+                            if (ev_row) ev_row[oi] = 0.0;
+                            den_row[oi] = 1.0;
+                        }
                     }
-                    if (g.lane == 0) {
-                        if (ev_row) ev_row[i1] = res;
-                        den_row[i1] = den;
+                    pc = last;
+                    if (ng) {
+                        team_sync<true>();
+                        double sums[2 * kProbeGroup + 4];
+                        sweep_exp_val_group(phi, k, probes, ng, g, red, sums);
+                        const double den = sums[2 * kProbeGroup];
+                        if (g.lane == 0) {
+#pragma unroll
+                            for (uint32_t p = 0; p < kProbeGroup; ++p) {
+                                if (p >= ng) break;
+                                const ProbeRec r = load_probe(probes + p);
+                                const double num_re = sums[2 * p] * r.wr - sums[2 * p + 1] * r.wi;
+                                double res = den > 0.0 ? num_re / den : 0.0;
+                                if (res > 1.0) res = 1.0;
+                                if (res < -1.0) res = -1.0;
+                                if (ev_row) ev_row[r.out] = res;
+                                den_row[r.out] = den;
+                            }
+                        }
+                        team_sync<true>();  // the list is rewritten by the next group
                     }
                     break;
                 }
@@ -359,13 +481,32 @@ struct AdjInterp {
                     break;
                 }
                 case CLIFFT_OP_EXP_VAL: {
-                    uint32_t xa, za;
-                    double ph_r, ph_i;
+                    // this probe and the EXP_VALs before it (walking down;
+                    // frame opcodes between them are no-ops here)
                     cptr<AdjArgs> A = opaque(Ag);
-                    const double c = cp(A->coeffs)[i1];
-                    const double den = den_row[i1];
-                    if (c != 0.0 && den > 0.0 && probe(opaque(Pg), i0, xa, za, ph_r, ph_i))
-                        sweep_pauli_acc(lam, (const AmpT*)phi, k, xa, za, c * ph_r / den, c * ph_i / den, g);
+                    cptr<DevProgram> P = opaque(Pg);
+                    uint32_t ng = 0, last = pc;
+                    for (uint32_t t = pc + 1; t-- > 0;) {
+                        const uint64_t u0 = code[4 * t], u1 = code[4 * t + 1];
+                        const uint32_t uop = (uint32_t)(u0 & 0xFF);
+                        if (uop <= CLIFFT_OP_FRAME_SWAP) continue;
+                        if (uop != CLIFFT_OP_EXP_VAL || ng == kProbeGroup) break;
+                        last = t;
+                        const uint32_t ci = (uint32_t)u1, oi = (uint32_t)(u1 >> 32);
+                        const double c = cp(A->coeffs)[oi];
+                        const double den = den_row[oi];
+                        uint32_t xa, za;
+                        double ph_r, ph_i;
+                        if (c != 0.0 && den > 0.0 && probe(P, ci, xa, za, ph_r, ph_i)) {
+                            if (g.lane == 0) store_probe(probes + ng, xa, za, oi, c * ph_r / den, c * ph_i / den);
+                            ++ng;
+                        }
+                    }
+                    pc = last;
+                    if (ng) {
+                        team_sync<true>();
+                        sweep_pauli_acc_group(lam, (const AmpT*)phi, k, probes, ng, g);
+                    }
                     break;
                 }
                 default: break;

@@ -19,4 +19,13 @@ struct AdjArgs {
     uint32_t amp_elems;       // 2^peak_rank
 };
 
+// Consecutive EXP_VAL probes share one pass over the state, at most
+// kProbeGroup at a time; their records sit in the team's LDS.
+constexpr uint32_t kProbeGroup = 8;
+struct ProbeRec {
+    uint32_t xa, za, out, pad;
+    double wr, wi;  // forward: the probe phase; backward: the lambda weight
+};
+constexpr uint32_t kProbeBytes = kProbeGroup * sizeof(ProbeRec);
+
 }  // namespace csvm

@@ -1741,7 +1741,7 @@ int clifft_svm_adjoint_create(clifft_svm_program* p, const uint32_t* param_instr
     a->args_dev = reinterpret_cast<AdjArgs*>(static_cast<uint8_t*>(a->dev) + o_args);
     // team shape: phi and lambda in LDS up to rank 12, else global slices
     const uint32_t k = p->peak_rank;
-    const size_t red_dens = 16 * 4 * sizeof(double) + align_up((size_t)std::max<uint32_t>(p->num_ev, 1) * 8, 16);
+    const size_t red_dens = 16 * 4 * sizeof(double) + kProbeBytes + align_up((size_t)std::max<uint32_t>(p->num_ev, 1) * 8, 16);
     a->global_amps = k > 12 || red_dens + ((size_t)32 << k) > 160 * 1024;
     a->block = k <= 6 ? 64 : (k <= 8 ? 256 : (k <= 10 ? 512 : 1024));
     if (a->global_amps) {  // CLIFFT_ADJOINT_TEAM_THREADS: 256, 512 or 1024

@@ -421,6 +421,8 @@ __global__ void __launch_bounds__(1024) adjoint_kernel(const DevProgram* P, cons
     S.lam = S.phi + n;
     S.red = (lds_f64*)(p);
     p += 16 * 4 * sizeof(double);
+    S.probes = (lds_probe*)(p);
+    p += kProbeBytes;
     lds_f64* dens = (lds_f64*)(p);
     const uint64_t batch = Ag->batch;
     for (uint64_t b = blockIdx.x; b < batch; b += gridDim.x) {
