@@ -237,6 +237,7 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
 // workgroup) of this workgroup owns an LDS slice of lane_wave_bytes(); the
 // calling wavefront runs shot groups g = g_first, g_first + g_step, ...,
 // lane l taking shot s_lo + 64 g + l while that is below s_hi.
+template <bool kFrameFirst>
 __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint64_t g_first, uint64_t g_step,
                                            uint64_t s_lo, uint64_t s_hi, uint64_t seed) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -280,7 +281,7 @@ __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
             if (ev_row)
                 for (uint32_t i = 0; i < nev; ++i) ev_row[i] = 0.0;
 
-            S.run(Pg, ev_row);
+            S.template run<kFrameFirst>(Pg, ev_row);
 
             A = opaque(Ag);
             P = opaque(Pg);
@@ -329,7 +330,7 @@ __global__ void __launch_bounds__(256, MINB) lane_kernel(const DevProgram* P, co
     cptr<ShotArgs> Ag = cp(A);
     const uint64_t wpb = blockDim.x >> 6;
     const uint64_t wave_global = (uint64_t)blockIdx.x * wpb + uni(threadIdx.x >> 6);
-    lane_range(cp(P), Ag, wave_global, (uint64_t)gridDim.x * wpb, 0, Ag->shot_count, Ag->seed);
+    lane_range<MINB == 4>(cp(P), Ag, wave_global, (uint64_t)gridDim.x * wpb, 0, Ag->shot_count, Ag->seed);
 }
 
 // One program: teams stride over the launch's shots.
@@ -349,7 +350,7 @@ __global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram*
 // allocation (the save/restore at the call happens once per item).
 __device__ __noinline__ void batch_item_lane(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint32_t lo, uint32_t hi,
                                             uint64_t seed) {
-    lane_range(Pg, Ag, uni(threadIdx.x >> 6), blockDim.x >> 6, lo, hi, seed);
+    lane_range<false>(Pg, Ag, uni(threadIdx.x >> 6), blockDim.x >> 6, lo, hi, seed);
 }
 
 template <int FK, bool BT, bool GA>

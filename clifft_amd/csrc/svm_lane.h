@@ -321,6 +321,11 @@ struct LaneInterp {
     // ------------------------------------------------------------- dispatch
     // Runs the whole bytecode for this lane's shot; returns early (for this
     // lane only) on post-selection discard or an unreachable forced outcome.
+    // kFrameFirst: dispatch the frame opcodes (0..5) before the general
+    // switch.  Measured on one MI355X: 11% faster on the 20-qubit Clifford+T
+    // (peak 6, the 4-workgroup build), 2% slower on the surface-code memory
+    // (the 6-workgroup build), so only the 4-workgroup build uses it.
+    template <bool kFrameFirst>
     CSVM_INLINE void run(cptr<DevProgram> Pg, double* ev_row) {
         cptr<DevProgram> P0 = opaque(Pg);
         cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(P0->bc));
@@ -341,6 +346,14 @@ struct LaneInterp {
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
             if (!live) continue;
+            if (kFrameFirst && op <= CLIFFT_OP_FRAME_SWAP) {
+                if (op == CLIFFT_OP_FRAME_CNOT) frame_cnot(a1, a2);
+                else if (op == CLIFFT_OP_FRAME_CZ) frame_cz(a1, a2);
+                else if (op == CLIFFT_OP_FRAME_SWAP) frame_swap(a1, a2);
+                else if (op == CLIFFT_OP_FRAME_H) frame_h(a1);
+                else frame_s(a1, op == CLIFFT_OP_FRAME_S ? 1.0 : -1.0);
+                continue;
+            }
             switch (op) {
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
                 case CLIFFT_OP_FRAME_CZ: frame_cz(a1, a2); break;
