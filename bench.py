@@ -43,6 +43,8 @@ WORKLOADS = {
     "rz24_grad": ("24 qubits, 30 Clifford layers, 16 R_Z(theta_i) slots (peak k=16), 32 random Pauli probes "
                   "with nonzero expectation; "
                   "<O> and d<O>/dtheta for 512 theta vectors by the adjoint kernel", 512),
+    "programs4096": ("the 4096 programs of batch4096, 1000 shots each, one plain launch per program "
+                     "(the persistent batch launch is kept off the device)", 1000),
     "batch4096": ("4096 random Clifford+T programs, n ~ U[16,40], T-count ~ U[4,16], 40 layers, final M; "
                   "one persistent launch", 1000),
 }
@@ -288,6 +290,64 @@ def _measure_batch(progs, shots, seeds0, steps, warmup):
             "alg_bytes": alg, "tuning": tuning}
 
 
+def run_programs(args):
+    """Config: the batch4096 programs (4096 x 1000 shots) run with one plain
+    launch per program, back to back on one stream, records in device
+    memory."""
+    import ctypes
+
+    import torch
+    from clifft_amd import _lib, svm
+    from clifft_amd import workloads as W
+    from clifft_amd.compiler import compile as ccompile
+
+    desc, shots = WORKLOADS["programs4096"]
+    count = int(os.environ.get("CLIFFT_BENCH_BATCH_COUNT", "4096"))
+    if args.shots:
+        shots = args.shots
+    progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
+    seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
+    nb = lambda n: (n + 7) // 8
+    dev = torch.device("cuda:0")
+    bufs = [torch.empty((shots, nb(p.num_measurements)), dtype=torch.uint8, device=dev) for p in progs]
+    handles = [svm.device_program(p) for p in progs]
+    recs = (_lib.Records * count)(*[_lib.Records(b.data_ptr(), None, None, None, None) for b in bufs])
+    harr = (ctypes.c_void_p * count)(*[h.handle.value for h in handles])
+    shots_arr = np.full(count, shots, dtype=np.uint64)
+    flags = _lib.REC_DEVICE | _lib.REC_PACKED | _lib.BATCH_PER_PROGRAM
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def step(i):
+        _lib.check(_lib.lib().clifft_svm_batch_sample(harr, shots_arr.ctypes.data, (seeds0 + np.uint64(i)).ctypes.data,
+                                                      count, flags, recs, sh), "per-program launches")
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    alg = sum((h.info()["bytes_per_shot"] + nb(p.num_measurements)) * shots for h, p in zip(handles, progs))
+    achieved = alg / wall / 1e9
+    peaks = np.bincount([p.peak_rank for p in progs], minlength=17)
+    line = {
+        "metric": "shots/s", "value": shots * count / wall, "unit": "shots/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "programs4096", "description": desc, "programs": count, "shots_per_program": shots,
+                   "peak_rank_histogram": peaks.tolist(), "records": "bit-packed measurements, device memory",
+                   "parallelism": "shots within each program"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": wall * 1e3,
+                     "algorithmic_bytes_per_launch": alg, "note": "time is wall time of all launches of one step"},
+        "backend": _lib.backend(),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_batch(args):
     """Config: 4096 programs x 1000 shots in one persistent launch (autotuned),
     against one plain launch per program on the same stream.  Also reported:
@@ -367,6 +427,8 @@ def main():
         return
     if args.workload == "batch4096":
         return run_batch(args)
+    if args.workload == "programs4096":
+        return run_programs(args)
     if args.workload == "rz24_grad":
         return run_grad(args)
     desc, shots = WORKLOADS[args.workload]
