@@ -353,6 +353,21 @@ bool fuse_mono_enabled() {
     return v;
 }
 
+// Highest peak rank whose amplitudes stay in LDS in a workgroup team
+// (CLIFFT_LDS_MAX_RANK, 7..13; above it the global tier).  Measured on one
+// MI355X, per shot of random Clifford+T with 20000 shots: at peak 8 and 10
+// the LDS teams are faster (0.12 vs 0.28 us, 0.28 vs 0.35 us); at peak 11,
+// 12 and 13 the global tier with fused runs and small teams is 1.6x, 1.6x
+// and 3.1x faster (0.39 vs 0.64 us, 0.44 vs 0.70 us, 0.53 vs 1.66 us).
+int lds_max_rank() {
+    static int v = [] {
+        const char* e = std::getenv("CLIFFT_LDS_MAX_RANK");
+        const int r = e ? std::atoi(e) : 10;
+        return std::min(std::max(r, 7), 13);
+    }();
+    return v;
+}
+
 // Threads of a workgroup team on global amplitudes: CLIFFT_GLOBAL_TEAM_THREADS
 // (256, 512 or 1024) fixes it; 0 (unset) lets finalize_shape pick it from
 // the shot count.
@@ -393,7 +408,7 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
         L.block_team = false;
         L.block = 256;
         L.global_amps = false;
-    } else if (k <= 13) {
+    } else if ((int)k <= lds_max_rank()) {
         L.block_team = true;
         uint32_t waves = k <= 9 ? 2 : (k <= 10 ? 4 : (k <= 12 ? 8 : 16));
         L.block = 64 * waves;
