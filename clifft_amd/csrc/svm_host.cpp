@@ -290,6 +290,7 @@ struct clifft_svm_program {
     bool has_postselection = false;
     bool has_forced_ops = false;  // bytecode already contains *_FORCED opcodes
     bool has_mono_runs = false;   // some run of >= 2 permutation/phase opcodes could be fused
+    bool frame32 = false;         // every frame axis and frame mask fits 32 bits
     uint64_t num_array_ops = 0;
     double bytes_per_shot = 0.0;
     // importance sampling: per-site fault probabilities (quantum sites, then
@@ -398,6 +399,7 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
             // residency anyway, fewer registers only add spills (an
             // 8-workgroup build was 9% slower on 20-qubit Clifford+T, peak 6).
             L.lane_dense = 6 * L.lds_bytes <= 160 * 1024;
+            L.lane_narrow = p->frame32;
             return L;
         }
     }
@@ -613,6 +615,17 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     for (uint32_t e = 0; e < view->num_readout; ++e) p->fault_probs.push_back(view->ro_prob[e]);
     if (view->expected_observables)
         p->expected_obs.assign(view->expected_observables, view->expected_observables + view->num_expected_observables);
+    // 32-bit frame words suffice when no operand axis and no Pauli/noise
+    // mask reaches bit 32: the frame's upper bits then stay zero.
+    {
+        bool fits = view->words == 1;
+        for (uint64_t i = 0; i < view->num_instructions && fits; ++i)
+            fits = view->bytecode[i].axis_1 < 32 && view->bytecode[i].axis_2 < 32;
+        uint64_t any = 0;
+        for (uint32_t i = 0; fits && i < view->num_pauli_masks; ++i) any |= view->pauli_x[i] | view->pauli_z[i];
+        for (uint32_t i = 0; fits && i < view->num_channels; ++i) any |= view->chan_x[i] | view->chan_z[i];
+        p->frame32 = fits && (any >> 32) == 0;
+    }
     const uint32_t fbits = view->words * 64;
     uint32_t run = 0;
     for (uint64_t i = 0; i < view->num_instructions; ++i) {

@@ -236,8 +236,9 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
 // Lane tier: one shot per lane.  Wavefront `wave` (0 <= wave < waves in the
 // workgroup) of this workgroup owns an LDS slice of lane_wave_bytes(); the
 // calling wavefront runs shot groups g = g_first, g_first + g_step, ...,
-// lane l taking shot s_lo + 64 g + l while that is below s_hi.
-template <bool kFrameFirst>
+// lane l taking shot s_lo + 64 g + l while that is below s_hi.  FW is the
+// frame word (uint32_t for programs of at most 32 qubits).
+template <bool kFrameFirst, class FW>
 __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint64_t g_first, uint64_t g_step,
                                            uint64_t s_lo, uint64_t s_hi, uint64_t seed) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -247,7 +248,7 @@ __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
     const uint32_t peak = P->peak_rank, slots = P->total_meas_slots, ndet = P->num_det, nobs = P->num_obs;
     const uint32_t wm = lane_words(slots), wd = lane_words(ndet), nwords = wm + wd + lane_words(nobs);
     unsigned char* base = smem + (size_t)wave * lane_wave_bytes(peak, slots, ndet, nobs);
-    LaneInterp S;
+    LaneInterp<FW> S;
     S.amp = (lds_col_d2*)(base) + lane;
     lds_col_u32* recs = (lds_col_u32*)(base + (64u * 16u << peak)) + lane;
     S.rm = recs;
@@ -263,8 +264,7 @@ __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
             const uint32_t mode = A->mode;
             for (uint32_t w = 0; w < nwords; ++w) recs[w << 6] = 0u;
             S.px = S.pz = 0;
-            S.g_re = 1.0;
-            S.g_im = 0.0;
+            S.g_mag = 1.0;
             S.k = 0;
             S.dust = 0;
             S.discarded = 0;
@@ -306,7 +306,7 @@ __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
                     }
                 } else {
                     for (uint32_t i = 0; i < n; ++i) {
-                        uint32_t v = LaneInterp::rget(src, i);
+                        uint32_t v = LaneInterp<FW>::rget(src, i);
                         if (xor_expected && i < n_exp) v ^= load_u8(expo, i) ? 1u : 0u;
                         out[s * n + i] = (uint8_t)v;
                     }
@@ -325,12 +325,13 @@ __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
 // MINB = resident workgroups of 256 the register budget is sized for: 4
 // (128 VGPRs, no spills) for programs whose LDS columns limit occupancy
 // anyway, 6 (80 VGPRs, a few spills) when LDS would allow more waves.
-template <int MINB>
+// FW: 32-bit frame words for programs of at most 32 qubits, else 64-bit.
+template <int MINB, class FW>
 __global__ void __launch_bounds__(256, MINB) lane_kernel(const DevProgram* P, const ShotArgs* A) {
     cptr<ShotArgs> Ag = cp(A);
     const uint64_t wpb = blockDim.x >> 6;
     const uint64_t wave_global = (uint64_t)blockIdx.x * wpb + uni(threadIdx.x >> 6);
-    lane_range<MINB == 4>(cp(P), Ag, wave_global, (uint64_t)gridDim.x * wpb, 0, Ag->shot_count, Ag->seed);
+    lane_range<MINB == 4, FW>(cp(P), Ag, wave_global, (uint64_t)gridDim.x * wpb, 0, Ag->shot_count, Ag->seed);
 }
 
 // One program: teams stride over the launch's shots.
@@ -350,7 +351,7 @@ __global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram*
 // allocation (the save/restore at the call happens once per item).
 __device__ __noinline__ void batch_item_lane(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint32_t lo, uint32_t hi,
                                             uint64_t seed) {
-    lane_range<false>(Pg, Ag, uni(threadIdx.x >> 6), blockDim.x >> 6, lo, hi, seed);
+    lane_range<false, uint64_t>(Pg, Ag, uni(threadIdx.x >> 6), blockDim.x >> 6, lo, hi, seed);
 }
 
 template <int FK, bool BT, bool GA>
@@ -480,21 +481,30 @@ static hipError_t launch_one(const LaunchShape& L, const DevProgram* P, const Sh
     return hipGetLastError();
 }
 
+template <class FW>
+static const void* lane_fn_fw(const LaunchShape& L) {
+    return L.lane_dense ? reinterpret_cast<const void*>(lane_kernel<6, FW>)
+                        : reinterpret_cast<const void*>(lane_kernel<4, FW>);
+}
+
 static const void* lane_fn(const LaunchShape& L) {
-    return L.lane_dense ? reinterpret_cast<const void*>(lane_kernel<6>) : reinterpret_cast<const void*>(lane_kernel<4>);
+    return L.lane_narrow ? lane_fn_fw<uint32_t>(L) : lane_fn_fw<uint64_t>(L);
 }
 
 static hipError_t launch_lane(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st) {
-    static std::atomic<bool> attr_done[2] = {{false}, {false}};
-    if (!attr_done[L.lane_dense].load(std::memory_order_acquire)) {
+    static std::atomic<bool> attr_done[4] = {{false}, {false}, {false}, {false}};
+    const int v = (L.lane_dense ? 1 : 0) + (L.lane_narrow ? 2 : 0);
+    if (!attr_done[v].load(std::memory_order_acquire)) {
         hipError_t e = hipFuncSetAttribute(lane_fn(L), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
-        attr_done[L.lane_dense].store(true, std::memory_order_release);
+        attr_done[v].store(true, std::memory_order_release);
     }
-    if (L.lane_dense)
-        hipLaunchKernelGGL(lane_kernel<6>, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
-    else
-        hipLaunchKernelGGL(lane_kernel<4>, dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A);
+    switch (v) {
+        case 0: hipLaunchKernelGGL((lane_kernel<4, uint64_t>), dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A); break;
+        case 1: hipLaunchKernelGGL((lane_kernel<6, uint64_t>), dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A); break;
+        case 2: hipLaunchKernelGGL((lane_kernel<4, uint32_t>), dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A); break;
+        default: hipLaunchKernelGGL((lane_kernel<6, uint32_t>), dim3(L.grid), dim3(L.block), L.lds_bytes, st, P, A); break;
+    }
     return hipGetLastError();
 }
 

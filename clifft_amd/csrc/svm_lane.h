@@ -4,9 +4,9 @@
 // The bytecode walk (opcode fetch, operands, table addresses, the static
 // active rank k) is wavefront-uniform and lives in scalar registers; every
 // piece of per-shot state is a per-lane value:
-//   * Pauli frame (p_x, p_z) for programs of at most 64 qubits: two 64-bit
-//     values per lane;
-//   * gamma, the xoshiro256++ state, the next-noise index, discard/forced
+//   * Pauli frame (p_x, p_z): two 32-bit values per lane for programs of at
+//     most 32 qubits, two 64-bit values up to 64 qubits;
+//   * |gamma|, the xoshiro256++ state, the next-noise index, discard/forced
 //     bookkeeping;
 //   * the 2^k amplitudes in LDS, column-major over the wavefront
 //     (amplitude i of lane l at [i * 64 + l]), so the uniform loops of the
@@ -20,7 +20,14 @@
 //
 // Handler semantics are those of the team interpreter (svm_interp.h), which
 // cites the reference opcode handlers (src/clifft/svm/svm_kernels.inl,
-// svm_forced_kernels.cc, svm.h:177-248, svm_internal.h:65-79).
+// svm_forced_kernels.cc, svm.h:177-248, svm_internal.h:65-79), with one
+// difference: this tier writes records, detector/observable rows,
+// expectation values and log-probabilities, none of which depends on the
+// phase of gamma (expectation values are normalised by the array's own
+// norm, svm_kernels.inl:2031-2173).  Only |gamma| is kept: it decides when
+// the array is rescaled (svm.h:177-186).  Unit phases (+-1, +-i, the T
+// phases) leave it as it is; ROT weights and fused U2/U4 gamma multipliers
+// multiply it by their modulus.
 #pragma once
 
 #include "svm_device.h"
@@ -30,13 +37,14 @@ namespace csvm {
 typedef __attribute__((address_space(3))) dv2 lds_col_d2;
 typedef __attribute__((address_space(3))) uint32_t lds_col_u32;
 
+template <class FW>  // frame word: uint32_t (<= 32 qubits) or uint64_t
 struct LaneInterp {
     lds_col_d2* amp;   // this lane's column: amplitude i at amp[i << 6]
     lds_col_u32* rm;   // measurement bits (word w at rm[w << 6])
     lds_col_u32* rd;   // detector bits
     lds_col_u32* ro;   // observable bits
-    uint64_t px, pz;
-    double g_re, g_im;
+    FW px, pz;
+    double g_mag;  // |gamma|
     uint32_t k;  // wavefront-uniform
     Rng rng;
     uint32_t next_noise;
@@ -62,62 +70,45 @@ struct LaneInterp {
     // ----------------------------------------------------------------- frame
     CSVM_INLINE uint32_t gx(uint32_t q) const { return (uint32_t)(px >> q) & 1u; }
     CSVM_INLINE uint32_t gz(uint32_t q) const { return (uint32_t)(pz >> q) & 1u; }
-    CSVM_INLINE void set_x(uint32_t q, uint32_t b) { px = (px & ~(1ull << q)) | ((uint64_t)(b & 1u) << q); }
-    CSVM_INLINE void set_z(uint32_t q, uint32_t b) { pz = (pz & ~(1ull << q)) | ((uint64_t)(b & 1u) << q); }
-    CSVM_INLINE void mul_phase(double re, double im) {
-        double a = g_re, b = g_im;
-        g_re = a * re - b * im;
-        g_im = a * im + b * re;
-    }
-    CSVM_INLINE void negate_if(uint32_t c) {
-        double s = c ? -1.0 : 1.0;
-        g_re *= s;
-        g_im *= s;
-    }
+    CSVM_INLINE void set_x(uint32_t q, uint32_t b) { px = (px & ~((FW)1 << q)) | ((FW)(b & 1u) << q); }
+    CSVM_INLINE void set_z(uint32_t q, uint32_t b) { pz = (pz & ~((FW)1 << q)) | ((FW)(b & 1u) << q); }
+    // gamma *= (re + i im): only the modulus is kept
+    CSVM_INLINE void mul_gamma(double re, double im) { g_mag *= hypot(re, im); }
+    // Heisenberg frame updates (the phase they give gamma is not kept)
     CSVM_INLINE void frame_cnot(uint32_t c, uint32_t t) {
-        uint64_t xc = (px >> c) & 1ull, zt = (pz >> t) & 1ull;
+        const FW xc = (px >> c) & 1u, zt = (pz >> t) & 1u;
         px ^= xc << t;
         pz ^= zt << c;
     }
     CSVM_INLINE void frame_cz(uint32_t c, uint32_t t) {
-        uint64_t xc = (px >> c) & 1ull, xt = (px >> t) & 1ull;
-        negate_if((uint32_t)(xc & xt));
+        const FW xc = (px >> c) & 1u, xt = (px >> t) & 1u;
         pz ^= (xc << t) | (xt << c);
     }
     CSVM_INLINE void frame_h(uint32_t v) {
-        uint64_t x = (px >> v) & 1ull, z = (pz >> v) & 1ull;
-        negate_if((uint32_t)(x & z));
-        uint64_t d = (x ^ z) << v;
+        const FW d = (((px ^ pz) >> v) & 1u) << v;
         px ^= d;
         pz ^= d;
     }
-    CSVM_INLINE void frame_s(uint32_t v, double sgn) {
-        uint64_t x = (px >> v) & 1ull;
-        if (x) mul_phase(0.0, sgn);
-        pz ^= x << v;
-    }
+    CSVM_INLINE void frame_s(uint32_t v) { pz ^= ((px >> v) & 1u) << v; }
     CSVM_INLINE void frame_swap(uint32_t a, uint32_t b) {
-        uint64_t dx = ((px >> a) ^ (px >> b)) & 1ull, dz = ((pz >> a) ^ (pz >> b)) & 1ull;
+        const FW dx = ((px >> a) ^ (px >> b)) & 1u, dz = ((pz >> a) ^ (pz >> b)) & 1u;
         px ^= (dx << a) | (dx << b);
         pz ^= (dz << a) | (dz << b);
     }
-    CSVM_INLINE void apply_pauli(uint64_t ex, uint64_t ez, uint32_t sign) {
-        uint32_t par = (uint32_t)__popcll(px & ez) & 1u;
-        negate_if(par ^ (sign & 1u));
-        px ^= ex;
-        pz ^= ez;
+    // masks fit FW: every qubit index is below the program's qubit count
+    CSVM_INLINE void apply_pauli(uint64_t ex, uint64_t ez) {
+        px ^= (FW)ex;
+        pz ^= (FW)ez;
     }
 
     // ------------------------------------------------------------ magnitude
     CSVM_INLINE void scale_magnitude(double s) {
-        g_re *= s;
-        g_im *= s;
-        double gm = hypot(g_re, g_im);
+        g_mag *= s;
+        const double gm = g_mag;
         if (gm > 1e100 || (gm < 1e-100 && gm > 0.0)) {
             const uint32_t n = 1u << ku();
             for (uint32_t i = 0; i < n; ++i) S(i, cscale(A(i), gm));
-            g_re /= gm;
-            g_im /= gm;
+            g_mag = 1.0;
         }
     }
     CSVM_INLINE uint32_t sample_branch(double p0, double p1, double total) {
@@ -179,7 +170,7 @@ struct LaneInterp {
         for (uint32_t c = c0; c < c1; ++c) {
             cum += prob[c];
             if (r < cum) {
-                apply_pauli(P->chan_x[c], P->chan_z[c], 0);
+                apply_pauli(P->chan_x[c], P->chan_z[c]);
                 break;
             }
         }
@@ -208,7 +199,7 @@ struct LaneInterp {
         return true;
     }
     CSVM_INLINE bool meas_diagonal(uint32_t v, uint32_t idx, uint32_t sign, bool is_forced) {
-        const uint32_t x = gx(v), z = gz(v), half = 1u << (ku() - 1);
+        const uint32_t x = gx(v), half = 1u << (ku() - 1);
         double p0 = 0.0, p1 = 0.0;
         for (uint32_t i = 0; i < half; ++i) {
             dv2 a = A(i), b = A(i + half);
@@ -218,7 +209,6 @@ struct LaneInterp {
         double total = p0 + p1;
         uint32_t b, m_abs, rec;
         if (!choose(p0, p1, total, x, sign, idx, is_forced, b, m_abs, rec)) return false;
-        negate_if(b & z);
         if (b)
             for (uint32_t i = 0; i < half; ++i) S(i, A(i + half));
         k = ku() - 1;
@@ -230,7 +220,7 @@ struct LaneInterp {
         return true;
     }
     CSVM_INLINE bool meas_interfere(uint32_t v, uint32_t idx, uint32_t sign, bool is_forced) {
-        const uint32_t x = gx(v), z = gz(v), half = 1u << (ku() - 1);
+        const uint32_t z = gz(v), half = 1u << (ku() - 1);
         double p0 = 0.0, p1 = 0.0;
         for (uint32_t i = 0; i < half; ++i) {
             dv2 a = A(i), b = A(i + half);
@@ -246,7 +236,6 @@ struct LaneInterp {
             dv2 a = A(i), b = A(i + half);
             S(i, mk((a.x + s * b.x) * kInvSqrt2, (a.y + s * b.y) * kInvSqrt2));
         }
-        negate_if(x & m_abs);
         k = ku() - 1;
         double pb = bx ? p1 : p0;
         if (is_forced || pb > 0.0) scale_magnitude(sqrt(total / pb));
@@ -258,7 +247,7 @@ struct LaneInterp {
     CSVM_INLINE bool swap_meas_interfere(uint32_t fa, uint32_t t, uint32_t idx, uint32_t sign, bool is_forced) {
         if (fa == t) return meas_interfere(t, idx, sign, is_forced);
         frame_swap(fa, t);
-        const uint32_t x = gx(t), z = gz(t);
+        const uint32_t z = gz(t);
         const uint32_t half = 1u << (ku() - 1), top = ku() - 1, fb = 1u << fa, tb = 1u << top;
         double p0 = 0.0, p1 = 0.0;
         for (uint32_t j = 0; j < half; ++j) {
@@ -282,7 +271,6 @@ struct LaneInterp {
         k = ku() - 1;
         double pb = bx ? p1 : p0;
         if (is_forced || pb > 0.0) scale_magnitude(sqrt(total / pb));
-        negate_if(x & m_abs);
         set_x(t, m_abs);
         set_z(t, 0);
         rset(rm, idx, rec);
@@ -290,7 +278,7 @@ struct LaneInterp {
     }
 
     CSVM_INLINE double exp_val(uint64_t mx, uint64_t mz, uint32_t msign) {
-        uint32_t fsign = msign ^ ((uint32_t)(__popcll(mx & pz) + __popcll(mz & px)) & 1u);
+        uint32_t fsign = msign ^ ((uint32_t)(__popcll(mx & (uint64_t)pz) + __popcll(mz & (uint64_t)px)) & 1u);
         const uint64_t am = (1ull << ku()) - 1ull;
         if (mx & ~am) return 0.0;  // X/Y support on a dormant qubit
         const uint32_t xa = (uint32_t)(mx & am), za = (uint32_t)(mz & am);
@@ -351,15 +339,15 @@ struct LaneInterp {
                 else if (op == CLIFFT_OP_FRAME_CZ) frame_cz(a1, a2);
                 else if (op == CLIFFT_OP_FRAME_SWAP) frame_swap(a1, a2);
                 else if (op == CLIFFT_OP_FRAME_H) frame_h(a1);
-                else frame_s(a1, op == CLIFFT_OP_FRAME_S ? 1.0 : -1.0);
+                else frame_s(a1);
                 continue;
             }
             switch (op) {
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
                 case CLIFFT_OP_FRAME_CZ: frame_cz(a1, a2); break;
                 case CLIFFT_OP_FRAME_H: frame_h(a1); break;
-                case CLIFFT_OP_FRAME_S: frame_s(a1, 1.0); break;
-                case CLIFFT_OP_FRAME_S_DAG: frame_s(a1, -1.0); break;
+                case CLIFFT_OP_FRAME_S:
+                case CLIFFT_OP_FRAME_S_DAG: frame_s(a1); break;
                 case CLIFFT_OP_FRAME_SWAP: frame_swap(a1, a2); break;
                 case CLIFFT_OP_ARRAY_CNOT: {
                     const uint32_t n = 1u << (ku() - 2), ab = 1u << a1, bb = 1u << a2;
@@ -449,7 +437,7 @@ struct LaneInterp {
                         uint32_t x1 = insert_zero_bit(j, a1) | vb;
                         S(x1, cmul(A(x1), ph));
                     }
-                    frame_s(a1, sg);
+                    frame_s(a1);
                     break;
                 }
                 case CLIFFT_OP_ARRAY_T:
@@ -469,14 +457,14 @@ struct LaneInterp {
                             S(x1, cmul(A(x1), ph));
                         }
                     }
-                    if (x) mul_phase(zr, zi);
+                    if (x && op == CLIFFT_OP_ARRAY_ROT) mul_gamma(zr, zi);
                     break;
                 }
                 case CLIFFT_OP_ARRAY_U2: {
                     cptr<DevProgram> P = opaque(Pg);
                     const clifft_u2_node* node = P->u2 + i0;
                     const uint32_t s = (gz(a1) << 1) | gx(a1);
-                    mul_phase(node->gamma[s][0], node->gamma[s][1]);
+                    mul_gamma(node->gamma[s][0], node->gamma[s][1]);
                     const uint32_t out = node->out_states[s];
                     set_x(a1, out & 1u);
                     set_z(a1, (out >> 1) & 1u);
@@ -498,7 +486,7 @@ struct LaneInterp {
                     cptr<DevProgram> P = opaque(Pg);
                     const uint32_t s = (gz(a2) << 3) | (gx(a2) << 2) | (gz(a1) << 1) | gx(a1);
                     const clifft_u4_entry* e = &P->u4[i0].entries[s];
-                    mul_phase(e->gamma[0], e->gamma[1]);
+                    mul_gamma(e->gamma[0], e->gamma[1]);
                     const uint32_t out = e->out_state;
                     set_x(a1, out & 1u);
                     set_z(a1, (out >> 1) & 1u);
@@ -542,7 +530,7 @@ struct LaneInterp {
                     for (uint32_t i = 0; i < half; ++i) S(i + half, has_phase ? cmul(A(i), ph) : A(i));
                     k = ku() + 1;
                     scale_magnitude(1.0 / sqrt(2.0));
-                    if (has_phase && x) mul_phase(zr, zi);
+                    if (x && op == CLIFFT_OP_EXPAND_ROT) mul_gamma(zr, zi);
                     break;
                 }
                 case CLIFFT_OP_MEAS_DORMANT_STATIC:
@@ -550,7 +538,6 @@ struct LaneInterp {
                     break;
                 case CLIFFT_OP_MEAS_DORMANT_RANDOM: {
                     const uint32_t m = rng_double(rng) < 0.5 ? 0u : 1u;
-                    negate_if(gx(a1) & m);
                     set_x(a1, m);
                     set_z(a1, 0);
                     rset(rm, i0, m ^ sign);
@@ -573,7 +560,6 @@ struct LaneInterp {
                     const uint32_t req = forced[i0] ? 1u : 0u;
                     const uint32_t m = req ^ sign;
                     logp += log(0.5);
-                    negate_if(gx(a1) & m);
                     set_x(a1, m);
                     set_z(a1, 0);
                     rset(rm, i0, req);
@@ -591,7 +577,7 @@ struct LaneInterp {
                 case CLIFFT_OP_APPLY_PAULI:
                     if (rget(rm, i1)) {
                         cptr<DevProgram> P = opaque(Pg);
-                        apply_pauli(cp(P->pauli_x)[i0], cp(P->pauli_z)[i0], load_u8(cp(P->pauli_sign), i0) ? 1u : 0u);
+                        apply_pauli(cp(P->pauli_x)[i0], cp(P->pauli_z)[i0]);
                     }
                     break;
                 case CLIFFT_OP_NOISE:
