@@ -55,7 +55,9 @@ struct DevProgram {
     const uint64_t* chan_x;
     const uint64_t* chan_z;
     const double* chan_prob;
+    const double* chan_cum;   // running sum of chan_prob within each site
     const double* hazards;
+    const uint32_t* hz_guide;  // hz_buckets + 1 entries (see hazard_upper_bound)
     const uint32_t* ro_meas_idx;
     const double* ro_prob;
     const uint32_t* det_off;
@@ -75,6 +77,9 @@ struct DevProgram {
     uint32_t num_ev;
     uint32_t num_sites;
     uint32_t n_expected_obs;
+    uint32_t hz_buckets;  // 0: no guide (no sites or zero total hazard)
+    uint32_t pad0;
+    double hz_scale;      // hz_buckets / hazards[num_sites - 1]
 };
 
 // Host-visible state blob for single-shot execution from a preset state.
@@ -261,6 +266,57 @@ CSVM_INLINE dv2 cadd(dv2 a, dv2 b) { return a + b; }
 CSVM_INLINE dv2 csub(dv2 a, dv2 b) { return a - b; }
 CSVM_INLINE dv2 cscale(dv2 a, double s) { return a * s; }
 CSVM_INLINE dv2 cneg(dv2 a) { return -a; }
+
+// ---------------------------------------------------------------------------
+// Noise-site search (svm.h:229-243, svm_kernels.inl:1886-1920)
+// ---------------------------------------------------------------------------
+// First i in [lo, hi) with hz[i] > target, hi if none (hz non-decreasing).
+template <class PD>
+CSVM_INLINE uint32_t upper_bound_in(PD hz, uint32_t lo, uint32_t hi, double target) {
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (hz[mid] <= target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// std::upper_bound(hz, hz + n, target) for n >= 1 (the gap sampler's next
+// firing site).  The guide holds, for G buckets of width H / G over
+// [0, H = hz[n-1]), the upper bound of each bucket's lower edge
+// (guide[G] = n).  The search runs over the sites within one bucket either
+// side of the target's bucket; the answer is checked against its two
+// neighbours and, should the bucket index have rounded past the window,
+// replaced by a search of the whole table.  The result is upper_bound's in
+// every case.
+template <class PD, class PU>
+CSVM_INLINE uint32_t hazard_upper_bound(PD hz, uint32_t n, PU guide, uint32_t G, double scale, double target) {
+    if (!(target < hz[n - 1])) return n;
+    if (G == 0) return upper_bound_in(hz, 0, n, target);
+    const double fb = target * scale;
+    const uint32_t b = fb < (double)G ? (uint32_t)fb : G - 1u;
+    uint32_t lo = upper_bound_in(hz, guide[b > 0 ? b - 1u : 0u], guide[b + 2u < G ? b + 2u : G], target);
+    if ((lo > 0 && hz[lo - 1] > target) || (lo < n && hz[lo] <= target)) lo = upper_bound_in(hz, 0, n, target);
+    return lo;
+}
+
+// First channel c in [c0, c1) with r < cum[c], c1 if none: the reference's
+// running-sum scan, on running sums precomputed in the same order (so the
+// same doubles); a binary search, as the sums never decrease.
+template <class PD>
+CSVM_INLINE uint32_t pick_channel(PD cum, uint32_t c0, uint32_t c1, double r) {
+    uint32_t lo = c0, hi = c1;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (r < cum[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
 
 // svm_math.h:58-61
 CSVM_INLINE uint32_t insert_zero_bit(uint32_t val, uint32_t pos) {

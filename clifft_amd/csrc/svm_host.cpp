@@ -106,6 +106,11 @@ int validate(const clifft_program_view* v, std::vector<int32_t>& khist, bool& ha
         if (!(h >= 0.0) || !std::isfinite(h) || (s > 0 && h < v->hazards[s - 1]))
             return fail(CLIFFT_E_INVALID, "noise hazard table must be finite and non-decreasing");
     }
+    // Channel probabilities are probabilities: the channel search relies on
+    // their running sums never decreasing.
+    for (uint32_t c = 0; c < v->num_channels; ++c)
+        if (!(v->chan_prob[c] >= 0.0) || !std::isfinite(v->chan_prob[c]))
+            return fail(CLIFFT_E_INVALID, "noise channel probabilities must be finite and non-negative");
     for (uint32_t i = 0; i < v->num_readout; ++i)
         if (v->ro_meas_idx[i] >= v->total_meas_slots) return fail(CLIFFT_E_INVALID, "readout meas_idx out of range");
     if (v->num_det_lists > 0 && v->det_off[v->num_det_lists] > 0)
@@ -674,6 +679,28 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     size_t o_cz = add(view->chan_z, view->num_channels * W * 8);
     size_t o_cp = add(view->chan_prob, view->num_channels * 8);
     size_t o_hz = add(view->hazards, view->num_sites * 8);
+    // Running channel sums per site, in the reference's summation order
+    // (svm_kernels.inl:1894-1909), and the hazard guide: G buckets of width
+    // H / G over [0, H), H the total hazard; entry b is the upper bound of
+    // the bucket's lower edge, entry G is the site count.
+    std::vector<double> chan_cum(view->num_channels);
+    for (uint32_t s = 0; s < view->num_sites; ++s) {
+        double acc = 0.0;
+        for (uint32_t c = view->site_chan_off[s]; c < view->site_chan_off[s + 1]; ++c) {
+            acc += view->chan_prob[c];
+            chan_cum[c] = acc;
+        }
+    }
+    const uint32_t n_sites = view->num_sites;
+    const double h_total = n_sites ? view->hazards[n_sites - 1] : 0.0;
+    const uint32_t hz_g = h_total > 0.0 ? std::min<uint32_t>(n_sites, 1u << 16) : 0u;
+    const double hz_scale = hz_g ? (double)hz_g / h_total : 0.0;
+    std::vector<uint32_t> hz_guide(hz_g + 1, n_sites);
+    for (uint32_t b = 0; b < hz_g; ++b)
+        hz_guide[b] = (uint32_t)(std::upper_bound(view->hazards, view->hazards + n_sites, (double)b / hz_scale) -
+                                 view->hazards);
+    size_t o_cc = add(chan_cum.data(), chan_cum.size() * 8);
+    size_t o_hg = add(hz_guide.data(), hz_guide.size() * 4);
     size_t o_rm = add(view->ro_meas_idx, view->num_readout * 4);
     size_t o_rp = add(view->ro_prob, view->num_readout * 8);
     size_t o_do = add(view->det_off, view->num_det_lists > 0 ? (view->num_det_lists + 1) * 4 : 0);
@@ -722,7 +749,11 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     d.chan_x = reinterpret_cast<const uint64_t*>(base + o_cx);
     d.chan_z = reinterpret_cast<const uint64_t*>(base + o_cz);
     d.chan_prob = reinterpret_cast<const double*>(base + o_cp);
+    d.chan_cum = reinterpret_cast<const double*>(base + o_cc);
     d.hazards = reinterpret_cast<const double*>(base + o_hz);
+    d.hz_guide = reinterpret_cast<const uint32_t*>(base + o_hg);
+    d.hz_buckets = hz_g;
+    d.hz_scale = hz_scale;
     d.ro_meas_idx = reinterpret_cast<const uint32_t*>(base + o_rm);
     d.ro_prob = reinterpret_cast<const double*>(base + o_rp);
     d.det_off = reinterpret_cast<const uint32_t*>(base + o_do);

@@ -283,33 +283,20 @@ struct Interp {
         cptr<double> hz = cp(P->hazards);
         double cur = next_noise == 0 ? 0.0 : hz[next_noise - 1];
         double gap = -log(1.0 - rng_double(rng));
-        double target = cur + gap;
-        uint32_t lo = 0, hi = n;
-        while (lo < hi) {
-            uint32_t mid = lo + (hi - lo) / 2;
-            if (hz[mid] <= target)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        next_noise = lo;
+        next_noise = uni(hazard_upper_bound(hz, n, cp(P->hz_guide), P->hz_buckets, P->hz_scale, cur + gap));
     }
     // :1886-1920
     CSVM_INLINE void noise(cptr<DevProgram> P, uint32_t site) {
         if (site != next_noise) return;
         cptr<uint32_t> off = cp(P->site_chan_off);
-        cptr<double> prob = cp(P->chan_prob);
-        uint32_t c0 = off[site], c1 = off[site + 1];
-        double psum = 0.0;
-        for (uint32_t c = c0; c < c1; ++c) psum += prob[c];
-        double r = rng_double(rng) * psum;
-        double cum = 0.0;
-        for (uint32_t c = c0; c < c1; ++c) {
-            cum += prob[c];
-            if (r < cum) {
+        cptr<double> cum = cp(P->chan_cum);
+        const uint32_t c0 = off[site], c1 = off[site + 1];
+        const double u = rng_double(rng);
+        if (c1 > c0) {
+            const uint32_t c = uni(pick_channel(cum, c0, c1, u * cum[c1 - 1]));
+            if (c < c1) {
                 uint32_t W = P->words;
                 apply_pauli(cp(P->chan_x) + (size_t)c * W, cp(P->chan_z) + (size_t)c * W, W, 0);
-                break;
             }
         }
         if (kf) {  // svm.h:218-225

@@ -274,7 +274,7 @@ __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
             S.rng = rng_for_shot(seed, A->shot_begin + s);
             S.S(0, mk(1.0, 0.0));
             S.next_noise = 0;
-            S.draw_next_noise(P->hazards, P->num_sites);
+            S.draw_next_noise(P, 0.0);
             if (mode == MODE_FORCED) S.forced = A->forced_records + s * P->num_meas;
             const uint32_t nev = P->num_ev;
             double* ev_row = A->out_ev ? A->out_ev + s * nev : nullptr;

@@ -140,42 +140,31 @@ struct LaneInterp {
     }
 
     // ----------------------------------------------------------------- noise
-    CSVM_INLINE void draw_next_noise(const double* hz, uint32_t n) {
+    // Gap sampler (svm.h:229-243): the next firing site is the first whose
+    // cumulative hazard exceeds cur (the hazard through the last site
+    // reached) plus an Exp(1) gap.
+    CSVM_INLINE void draw_next_noise(cptr<DevProgram> P, double cur) {
+        const uint32_t n = P->num_sites;
         if (n == 0 || next_noise >= n) {
             next_noise = kNoNoise;
             return;
         }
-        double cur = next_noise == 0 ? 0.0 : hz[next_noise - 1];
-        double gap = -log(1.0 - rng_double(rng));
-        double target = cur + gap;
-        uint32_t lo = 0, hi = n;
-        while (lo < hi) {
-            uint32_t mid = lo + (hi - lo) / 2;
-            if (hz[mid] <= target)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        next_noise = lo;
+        const double gap = -log(1.0 - rng_double(rng));
+        next_noise = hazard_upper_bound(P->hazards, n, P->hz_guide, P->hz_buckets, P->hz_scale, cur + gap);
     }
-    // Fire the noise site `site` (== next_noise) and draw the next one.
+    // Fire the noise site `site` (== next_noise) and draw the next one
+    // (svm_kernels.inl:1886-1920).
     CSVM_INLINE void noise(cptr<DevProgram> P, uint32_t site) {
-        const uint32_t* off = P->site_chan_off;
-        const double* prob = P->chan_prob;
-        uint32_t c0 = off[site], c1 = off[site + 1];
-        double psum = 0.0;
-        for (uint32_t c = c0; c < c1; ++c) psum += prob[c];
-        double r = rng_double(rng) * psum;
-        double cum = 0.0;
-        for (uint32_t c = c0; c < c1; ++c) {
-            cum += prob[c];
-            if (r < cum) {
-                apply_pauli(P->chan_x[c], P->chan_z[c]);
-                break;
-            }
+        const uint32_t c0 = P->site_chan_off[site], c1 = P->site_chan_off[site + 1];
+        const double cur = P->hazards[site];
+        const double u = rng_double(rng);
+        if (c1 > c0) {
+            const double* cum = P->chan_cum;
+            const uint32_t c = pick_channel(cum, c0, c1, u * cum[c1 - 1]);
+            if (c < c1) apply_pauli(P->chan_x[c], P->chan_z[c]);
         }
         next_noise++;
-        draw_next_noise(P->hazards, P->num_sites);
+        draw_next_noise(P, cur);
     }
 
     // ---------------------------------------------------------- measurement
