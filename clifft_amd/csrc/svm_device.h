@@ -43,8 +43,12 @@ constexpr uint32_t kNoNoise = 0xFFFFFFFFu;
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
 // Device copy of a program (all pointers are device memory).
+// Marker (payload bytes 8..9) of a lane-bytecode record parity held inline.
+constexpr uint32_t kLaneInlineParity = 1;
+
 struct DevProgram {
     const clifft_instruction* bc;
+    const clifft_instruction* lane_bc;  // lane interpreter copy, operands inlined
     const uint64_t* pauli_x;
     const uint64_t* pauli_z;
     const uint8_t* pauli_sign;

@@ -330,6 +330,55 @@ size_t rec_stride_of(const clifft_svm_program* p, bool kfault = false) {
            (kfault ? 20u * kMaxForcedFaults : 0u);
 }
 
+// Operands the lane interpreter reads from the instruction itself (single
+// frame word programs), written into payload bytes 8..23 that these opcodes
+// do not use:
+//   APPLY_PAULI: the X and Z masks of the frame correction;
+//   DETECTOR, POSTSELECT, OBSERVABLE: the record parity as (word, mask)
+//     pairs over the 32-bit words of the measurement record -- bytes 8..9
+//     hold kLaneInlineParity, 10..11 word a, 12..15 mask a, 16..17 word b,
+//     20..23 mask b.  Targets listed twice cancel, as in the XOR they feed.
+//     Lists spanning more than two words keep the CSR tables (marker 0).
+void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code) {
+    for (uint64_t i = 0; i < v->num_instructions; ++i) {
+        clifft_instruction& in = code[i];
+        uint64_t w2 = 0, w3 = 0;
+        if (in.opcode == CLIFFT_OP_APPLY_PAULI) {
+            w2 = v->pauli_x[in.p.pauli.cp_mask_idx];
+            w3 = v->pauli_z[in.p.pauli.cp_mask_idx];
+        } else if (in.opcode == CLIFFT_OP_DETECTOR || in.opcode == CLIFFT_OP_POSTSELECT ||
+                   in.opcode == CLIFFT_OP_OBSERVABLE) {
+            const bool obs = in.opcode == CLIFFT_OP_OBSERVABLE;
+            const uint32_t* off = obs ? v->obs_off : v->det_off;
+            const uint32_t* tgt = obs ? v->obs_tgt : v->det_tgt;
+            const uint32_t l = in.p.classical.classical_idx;
+            std::vector<std::pair<uint32_t, uint32_t>> words;  // (word, mask)
+            for (uint32_t j = off[l]; j < off[l + 1]; ++j) {
+                const uint32_t w = tgt[j] >> 5, bit = 1u << (tgt[j] & 31u);
+                auto it = std::find_if(words.begin(), words.end(), [&](const auto& e) { return e.first == w; });
+                if (it == words.end())
+                    words.push_back({w, bit});
+                else
+                    it->second ^= bit;
+            }
+            words.erase(std::remove_if(words.begin(), words.end(), [](const auto& e) { return e.second == 0; }),
+                        words.end());
+            if (words.size() > 2 || (words.size() > 0 && words.back().first > 0xFFFF) ||
+                (words.size() > 1 && words[0].first > 0xFFFF))
+                continue;  // marker stays 0: CSR tables
+            uint32_t wa = 0, ma = 0, wb = 0, mb = 0;
+            if (words.size() > 0) wa = words[0].first, ma = words[0].second;
+            if (words.size() > 1) wb = words[1].first, mb = words[1].second;
+            w2 = (uint64_t)kLaneInlineParity | ((uint64_t)wa << 16) | ((uint64_t)ma << 32);
+            w3 = (uint64_t)wb | ((uint64_t)mb << 32);
+        } else {
+            continue;
+        }
+        std::memcpy(in.p.raw + 8, &w2, 8);
+        std::memcpy(in.p.raw + 16, &w3, 8);
+    }
+}
+
 // Highest peak rank run one shot per lane (CLIFFT_LANE_MAX_RANK overrides;
 // a negative value turns the lane tier off).  Measured on the 20-qubit,
 // peak-6 Clifford+T workload: lanes 1.5e8 shots/s, wavefront teams 1.3e7.
@@ -668,6 +717,8 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     const size_t W = view->words;
     size_t o_bc = add(view->bytecode, view->num_instructions * sizeof(clifft_instruction));
     size_t o_fbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
+    size_t o_lbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
+    size_t o_lfbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
     size_t o_px = add(view->pauli_x, view->num_pauli_masks * W * 8);
     size_t o_pz = add(view->pauli_z, view->num_pauli_masks * W * 8);
     size_t o_ps = add(view->pauli_sign, view->num_pauli_masks);
@@ -724,6 +775,14 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
             if (op >= CLIFFT_OP_MEAS_DORMANT_STATIC && op <= CLIFFT_OP_SWAP_MEAS_INTERFERE)
                 op = (uint8_t)(op + (CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED - CLIFFT_OP_MEAS_DORMANT_STATIC));
         }
+        // the lane interpreter's copies (sampling and forced), with operands
+        // inlined into the unused payload words
+        auto* lbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_lbc);
+        std::memcpy(lbc, view->bytecode, view->num_instructions * sizeof(clifft_instruction));
+        if (view->words == 1) lane_inline_operands(view, lbc);
+        auto* lfbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_lfbc);
+        std::memcpy(lfbc, lbc, view->num_instructions * sizeof(clifft_instruction));
+        for (uint64_t i = 0; i < view->num_instructions; ++i) lfbc[i].opcode = fbc[i].opcode;
     }
     hipError_t e = hipMalloc(&p->dev_blob, total);
     if (e == hipSuccess) e = hipMemcpy(p->dev_blob, staging.data(), total, hipMemcpyHostToDevice);
@@ -739,6 +798,7 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     DevProgram& d = p->dp;
     d.bc = reinterpret_cast<const clifft_instruction*>(base + o_bc);
     p->forced_bc = reinterpret_cast<clifft_instruction*>(base + o_fbc);
+    d.lane_bc = reinterpret_cast<const clifft_instruction*>(base + o_lbc);
     d.pauli_x = reinterpret_cast<const uint64_t*>(base + o_px);
     d.pauli_z = reinterpret_cast<const uint64_t*>(base + o_pz);
     d.pauli_sign = base + o_ps;
@@ -776,6 +836,7 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     {
         DevProgram both[2] = {d, d};
         both[1].bc = p->forced_bc;
+        both[1].lane_bc = reinterpret_cast<const clifft_instruction*>(base + o_lfbc);
         e = hipMemcpy(base + o_dp, both, sizeof(both), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             std::string msg = std::string("program upload: ") + hipGetErrorString(e);

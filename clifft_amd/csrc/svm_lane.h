@@ -61,6 +61,12 @@ struct LaneInterp {
 
     // ------------------------------------------------------------ records
     CSVM_INLINE static uint32_t rget(const lds_col_u32* r, uint32_t idx) { return (r[(idx >> 5) << 6] >> (idx & 31u)) & 1u; }
+    // parity of the measurement-record bits under two (word, mask) pairs
+    // held in an instruction's payload words 2 and 3
+    CSVM_INLINE uint32_t inline_parity(uint64_t w2, uint64_t w3) const {
+        const uint32_t wa = (uint32_t)(w2 >> 16) & 0xFFFFu, wb = (uint32_t)w3 & 0xFFFFu;
+        return (uint32_t)(__popc(rm[wa << 6] & (uint32_t)(w2 >> 32)) + __popc(rm[wb << 6] & (uint32_t)(w3 >> 32))) & 1u;
+    }
     CSVM_INLINE static void rset(lds_col_u32* r, uint32_t idx, uint32_t v) {
         lds_col_u32* w = r + ((idx >> 5) << 6);
         uint32_t bit = 1u << (idx & 31u);
@@ -305,7 +311,7 @@ struct LaneInterp {
     template <bool kFrameFirst>
     CSVM_INLINE void run(cptr<DevProgram> Pg, double* ev_row) {
         cptr<DevProgram> P0 = opaque(Pg);
-        cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(P0->bc));
+        cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(P0->lane_bc));
         const uint32_t n_instr = P0->n_instr;
         // A shot that ends early (post-selection discard, unreachable
         // forced record) clears `live` and skips the remaining opcodes
@@ -315,7 +321,7 @@ struct LaneInterp {
         // fetched with vector loads and dispatched under exec masks).
         uint32_t live = 1;
         for (uint32_t pc = 0; pc < n_instr; ++pc) {
-            const uint64_t w0 = code[4 * pc], w1 = code[4 * pc + 1];
+            const uint64_t w0 = code[4 * pc], w1 = code[4 * pc + 1], w2 = code[4 * pc + 2], w3 = code[4 * pc + 3];
             const uint32_t op = (uint32_t)(w0 & 0xFF);
             const uint32_t flags = (uint32_t)((w0 >> 16) & 0xFF);
             const uint32_t a1 = (uint32_t)((w0 >> 32) & 0xFFFF);
@@ -435,7 +441,7 @@ struct LaneInterp {
                     double zr = kInvSqrt2, zi = (op == CLIFFT_OP_ARRAY_T_DAG) ? -kInvSqrt2 : kInvSqrt2;
                     if (op == CLIFFT_OP_ARRAY_ROT) {
                         zr = __longlong_as_double((long long)w1);
-                        zi = __longlong_as_double((long long)code[4 * pc + 2]);
+                        zi = __longlong_as_double((long long)w2);
                     }
                     const uint32_t x = gx(a1);
                     if (a1 < ku()) {
@@ -510,7 +516,7 @@ struct LaneInterp {
                     double zr = kInvSqrt2, zi = (op == CLIFFT_OP_EXPAND_T_DAG) ? -kInvSqrt2 : kInvSqrt2;
                     if (op == CLIFFT_OP_EXPAND_ROT) {
                         zr = __longlong_as_double((long long)w1);
-                        zi = __longlong_as_double((long long)code[4 * pc + 2]);
+                        zi = __longlong_as_double((long long)w2);
                     }
                     const bool has_phase = op != CLIFFT_OP_EXPAND;
                     const uint32_t x = gx(a1);
@@ -563,11 +569,8 @@ struct LaneInterp {
                 case CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED:
                     if (!swap_meas_interfere(a1, a2, i0, sign, true)) live = 0;
                     break;
-                case CLIFFT_OP_APPLY_PAULI:
-                    if (rget(rm, i1)) {
-                        cptr<DevProgram> P = opaque(Pg);
-                        apply_pauli(cp(P->pauli_x)[i0], cp(P->pauli_z)[i0]);
-                    }
+                case CLIFFT_OP_APPLY_PAULI:  // masks inlined in payload words 2 and 3
+                    if (rget(rm, i1)) apply_pauli(w2, w3);
                     break;
                 case CLIFFT_OP_NOISE:
                     if (i0 == next_noise) noise(opaque(Pg), i0);
@@ -590,11 +593,15 @@ struct LaneInterp {
                 }
                 case CLIFFT_OP_DETECTOR:
                 case CLIFFT_OP_POSTSELECT: {
-                    cptr<DevProgram> P = opaque(Pg);
-                    cptr<uint32_t> off = cp(P->det_off), tgt = cp(P->det_tgt);
                     uint32_t par = (flags & CLIFFT_FLAG_EXPECTED_ONE) ? 1u : 0u;
-                    const uint32_t j1 = off[i0 + 1];
-                    for (uint32_t j = off[i0]; j < j1; ++j) par ^= rget(rm, tgt[j]);
+                    if ((uint32_t)(w2 & 0xFFFF) == kLaneInlineParity) {
+                        par ^= inline_parity(w2, w3);
+                    } else {
+                        cptr<DevProgram> P = opaque(Pg);
+                        cptr<uint32_t> off = cp(P->det_off), tgt = cp(P->det_tgt);
+                        const uint32_t j1 = off[i0 + 1];
+                        for (uint32_t j = off[i0]; j < j1; ++j) par ^= rget(rm, tgt[j]);
+                    }
                     if (op == CLIFFT_OP_DETECTOR) {
                         rset(rd, i1, par);
                     } else {
@@ -607,11 +614,15 @@ struct LaneInterp {
                     break;
                 }
                 case CLIFFT_OP_OBSERVABLE: {
-                    cptr<DevProgram> P = opaque(Pg);
-                    cptr<uint32_t> off = cp(P->obs_off), tgt = cp(P->obs_tgt);
                     uint32_t par = 0;
-                    const uint32_t j1 = off[i0 + 1];
-                    for (uint32_t j = off[i0]; j < j1; ++j) par ^= rget(rm, tgt[j]);
+                    if ((uint32_t)(w2 & 0xFFFF) == kLaneInlineParity) {
+                        par = inline_parity(w2, w3);
+                    } else {
+                        cptr<DevProgram> P = opaque(Pg);
+                        cptr<uint32_t> off = cp(P->obs_off), tgt = cp(P->obs_tgt);
+                        const uint32_t j1 = off[i0 + 1];
+                        for (uint32_t j = off[i0]; j < j1; ++j) par ^= rget(rm, tgt[j]);
+                    }
                     rset(ro, i1, rget(ro, i1) ^ par);
                     break;
                 }
