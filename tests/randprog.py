@@ -57,9 +57,12 @@ def random_pauli_bits(rng, n):
 
 
 def random_program(seed, num_qubits=10, peak=6, length=200, noise=True, postselect=False,
-                   exp_vals=True, u_nodes=True, readout=True, dormant_meas=True, p_array=0.5, classical=True):
+                   exp_vals=True, u_nodes=True, readout=True, dormant_meas=True, p_array=0.5, classical=True,
+                   wide_lists=False):
     """A random program of about `length` instructions on `num_qubits`
-    qubits whose active rank never exceeds `peak`."""
+    qubits whose active rank never exceeds `peak`.  wide_lists: detector and
+    observable target lists of 1-8 entries drawn with repetition (targets
+    listed twice cancel in the parity)."""
     rng = np.random.default_rng(seed)
     n = num_qubits
     bc = []
@@ -200,7 +203,10 @@ def random_program(seed, num_qubits=10, peak=6, length=200, noise=True, postsele
                 readout_entries.append((int(rng.choice(meas_written)), float(rng.uniform(0.01, 0.4))))
                 bc.append(make_readout_noise(len(readout_entries) - 1))
             elif c == 2:
-                tg = [int(t) for t in rng.choice(meas_written, size=min(3, len(meas_written)), replace=False)]
+                if wide_lists:
+                    tg = [int(t) for t in rng.choice(meas_written, size=int(rng.integers(1, 9)), replace=True)]
+                else:
+                    tg = [int(t) for t in rng.choice(meas_written, size=min(3, len(meas_written)), replace=False)]
                 det_lists.append(tg)
                 if postselect and rng.random() < 0.3:
                     bc.append(make_postselect(len(det_lists) - 1, n_det, bool(rng.random() < 0.5)))
@@ -208,7 +214,10 @@ def random_program(seed, num_qubits=10, peak=6, length=200, noise=True, postsele
                     bc.append(make_detector(len(det_lists) - 1, n_det, bool(rng.random() < 0.5)))
                 n_det += 1
             else:
-                tg = [int(t) for t in rng.choice(meas_written, size=min(2, len(meas_written)), replace=False)]
+                if wide_lists:
+                    tg = [int(t) for t in rng.choice(meas_written, size=int(rng.integers(1, 9)), replace=True)]
+                else:
+                    tg = [int(t) for t in rng.choice(meas_written, size=min(2, len(meas_written)), replace=False)]
                 obs_lists.append(tg)
                 bc.append(make_observable(len(obs_lists) - 1, int(rng.integers(0, 2))))
                 n_obs = max(n_obs, 2)

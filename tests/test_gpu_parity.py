@@ -54,6 +54,24 @@ def test_sample_bit_exact(svm, orc, nq, peak, seed):
     np.testing.assert_allclose(res.exp_vals, e, atol=ATOL, rtol=0)
 
 
+@pytest.mark.parametrize("nq,peak", [(8, 2), (40, 5), (20, 9)])
+def test_wide_parity_lists(svm, orc, nq, peak):
+    """Detector/observable lists of up to 8 targets with repeats, over
+    records of more than 64 slots: parities held inline in the lane
+    bytecode (one or two record words, repeats cancelled) and lists that
+    keep the CSR tables (three or more words)."""
+    announce("wide-lists", nq, peak)
+    p = random_program(4242 + peak, num_qubits=nq, peak=peak, length=900, wide_lists=True)
+    assert p.total_meas_slots > 64 and p.num_detectors > 0
+    res = svm.sample(p, 256, seed=9)
+    m, d, o, e, _ = orc.sample(p, 256, seed=9, per_shot=True)
+    assert np.array_equal(res.measurements, m)
+    assert np.array_equal(res.detectors, d)
+    assert np.array_equal(res.observables, o)
+    packed = svm.sample(p, 256, seed=9, packed=True)
+    assert np.array_equal(np.packbits(d, axis=1, bitorder="little"), packed.detectors)
+
+
 @pytest.mark.parametrize("peak", [2, 4, 9, 14])
 def test_packed_records(svm, peak):
     announce("packed", peak)
