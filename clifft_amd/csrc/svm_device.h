@@ -45,6 +45,13 @@ typedef double dv2 __attribute__((ext_vector_type(2)));
 // Device copy of a program (all pointers are device memory).
 // Marker (payload bytes 8..9) of a lane-bytecode record parity held inline.
 constexpr uint32_t kLaneInlineParity = 1;
+// Lane-bytecode opcode (outside the reference's range) of a packed run of
+// 2..12 consecutive frame opcodes: axis_1 = ops in the run, axis_2 = the
+// bytecode instructions it replaces minus one (skipped after it); payload
+// words 1..3 hold 4 ops each, 16 bits per op: opcode (bits 0-2), first axis
+// (3-8), second axis (9-14).
+constexpr uint32_t kLaneFrameRun = 0xF0;
+constexpr uint32_t kLaneFrameRunMax = 12;
 
 struct DevProgram {
     const clifft_instruction* bc;

@@ -340,6 +340,29 @@ size_t rec_stride_of(const clifft_svm_program* p, bool kfault = false) {
 //     20..23 mask b.  Targets listed twice cancel, as in the XOR they feed.
 //     Lists spanning more than two words keep the CSR tables (marker 0).
 void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code) {
+    // runs of frame opcodes become one packed instruction (first of the run)
+    const uint64_t n = v->num_instructions;
+    for (uint64_t i = 0; i < n;) {
+        uint64_t j = i;
+        while (j < n && j - i < kLaneFrameRunMax && code[j].opcode <= CLIFFT_OP_FRAME_SWAP && code[j].axis_1 < 64 &&
+               code[j].axis_2 < 64)
+            ++j;
+        if (j - i < 2) {
+            i = std::max(j, i + 1);
+            continue;
+        }
+        uint64_t w[3] = {0, 0, 0};
+        for (uint64_t t = i; t < j; ++t) {
+            const uint64_t f = (uint64_t)code[t].opcode | ((uint64_t)code[t].axis_1 << 3) | ((uint64_t)code[t].axis_2 << 9);
+            w[(t - i) / 4] |= f << (16 * ((t - i) % 4));
+        }
+        clifft_instruction& h = code[i];
+        h.opcode = (uint8_t)kLaneFrameRun;
+        h.axis_1 = (uint16_t)(j - i);
+        h.axis_2 = (uint16_t)(j - i - 1);
+        std::memcpy(h.p.raw, w, 24);
+        i = j;
+    }
     for (uint64_t i = 0; i < v->num_instructions; ++i) {
         clifft_instruction& in = code[i];
         uint64_t w2 = 0, w3 = 0;
@@ -782,7 +805,11 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         if (view->words == 1) lane_inline_operands(view, lbc);
         auto* lfbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_lfbc);
         std::memcpy(lfbc, lbc, view->num_instructions * sizeof(clifft_instruction));
-        for (uint64_t i = 0; i < view->num_instructions; ++i) lfbc[i].opcode = fbc[i].opcode;
+        for (uint64_t i = 0; i < view->num_instructions; ++i) {
+            uint8_t& op = lfbc[i].opcode;
+            if (op >= CLIFFT_OP_MEAS_DORMANT_STATIC && op <= CLIFFT_OP_SWAP_MEAS_INTERFERE)
+                op = (uint8_t)(op + (CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED - CLIFFT_OP_MEAS_DORMANT_STATIC));
+        }
     }
     hipError_t e = hipMalloc(&p->dev_blob, total);
     if (e == hipSuccess) e = hipMemcpy(p->dev_blob, staging.data(), total, hipMemcpyHostToDevice);

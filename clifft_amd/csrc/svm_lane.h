@@ -328,6 +328,25 @@ struct LaneInterp {
             const uint32_t a2 = (uint32_t)(w0 >> 48);
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
+            if (op == kLaneFrameRun) {
+                // A packed run of frame opcodes.  The skip over the run's
+                // instructions is taken by every lane (pc stays uniform);
+                // the frame updates only by live lanes.
+                if (live) {
+                    for (uint32_t j = 0; j < a1; ++j) {
+                        const uint64_t w = j < 4 ? w1 : (j < 8 ? w2 : w3);
+                        const uint32_t f = (uint32_t)(w >> (16 * (j & 3u)));
+                        const uint32_t fop = f & 7u, fa = (f >> 3) & 63u, fb = (f >> 9) & 63u;
+                        if (fop == CLIFFT_OP_FRAME_CNOT) frame_cnot(fa, fb);
+                        else if (fop == CLIFFT_OP_FRAME_CZ) frame_cz(fa, fb);
+                        else if (fop == CLIFFT_OP_FRAME_SWAP) frame_swap(fa, fb);
+                        else if (fop == CLIFFT_OP_FRAME_H) frame_h(fa);
+                        else frame_s(fa);
+                    }
+                }
+                pc += a2;
+                continue;
+            }
             if (!live) continue;
             if (kFrameFirst && op <= CLIFFT_OP_FRAME_SWAP) {
                 if (op == CLIFFT_OP_FRAME_CNOT) frame_cnot(a1, a2);
