@@ -52,6 +52,13 @@ constexpr uint32_t kLaneInlineParity = 1;
 // (3-8), second axis (9-14).
 constexpr uint32_t kLaneFrameRun = 0xF0;
 constexpr uint32_t kLaneFrameRunMax = 12;
+// Lane-bytecode opcode of a dormant measurement (MEAS_DORMANT_STATIC or
+// MEAS_DORMANT_RANDOM, kept in the reserved byte) followed by an
+// APPLY_PAULI conditioned on that measurement's record: payload words 2 and
+// 3 hold the APPLY_PAULI masks, axis_2 = 1 (the APPLY_PAULI is skipped).
+// Lane superinstructions are the opcodes from kLaneFrameRun up; each names
+// in axis_2 how many instructions after it it covers.
+constexpr uint32_t kLaneMeasPauli = 0xF1;
 
 struct DevProgram {
     const clifft_instruction* bc;

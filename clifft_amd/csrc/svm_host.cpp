@@ -402,6 +402,24 @@ void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code
     }
 }
 
+// A dormant measurement immediately followed by the APPLY_PAULI that its
+// record conditions (the reset after a measurement) becomes one lane
+// instruction (kLaneMeasPauli); the APPLY_PAULI's masks are already inline.
+void lane_fuse_meas_pauli(const clifft_program_view* v, clifft_instruction* code) {
+    for (uint64_t i = 0; i + 1 < v->num_instructions; ++i) {
+        clifft_instruction& m = code[i];
+        const clifft_instruction& a = code[i + 1];
+        if ((m.opcode != CLIFFT_OP_MEAS_DORMANT_STATIC && m.opcode != CLIFFT_OP_MEAS_DORMANT_RANDOM) ||
+            a.opcode != CLIFFT_OP_APPLY_PAULI || a.p.pauli.condition_idx != m.p.classical.classical_idx)
+            continue;
+        m.reserved = m.opcode;
+        m.opcode = (uint8_t)kLaneMeasPauli;
+        m.axis_2 = 1;
+        std::memcpy(m.p.raw + 8, a.p.raw + 8, 16);
+        ++i;
+    }
+}
+
 // Highest peak rank run one shot per lane (CLIFFT_LANE_MAX_RANK overrides;
 // a negative value turns the lane tier off).  Measured on the 20-qubit,
 // peak-6 Clifford+T workload: lanes 1.5e8 shots/s, wavefront teams 1.3e7.
@@ -810,6 +828,7 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
             if (op >= CLIFFT_OP_MEAS_DORMANT_STATIC && op <= CLIFFT_OP_SWAP_MEAS_INTERFERE)
                 op = (uint8_t)(op + (CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED - CLIFFT_OP_MEAS_DORMANT_STATIC));
         }
+        if (view->words == 1) lane_fuse_meas_pauli(view, lbc);  // sampling copy only
     }
     hipError_t e = hipMalloc(&p->dev_blob, total);
     if (e == hipSuccess) e = hipMemcpy(p->dev_blob, staging.data(), total, hipMemcpyHostToDevice);

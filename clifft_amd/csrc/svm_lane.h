@@ -320,7 +320,8 @@ struct LaneInterp {
         // scalar unit (a per-lane exit would make them per-lane values,
         // fetched with vector loads and dispatched under exec masks).
         uint32_t live = 1;
-        for (uint32_t pc = 0; pc < n_instr; ++pc) {
+        uint32_t skip = 0;  // instructions a lane superinstruction covers beyond its own
+        for (uint32_t pc = 0; pc < n_instr; pc += 1 + skip) {
             const uint64_t w0 = code[4 * pc], w1 = code[4 * pc + 1], w2 = code[4 * pc + 2], w3 = code[4 * pc + 3];
             const uint32_t op = (uint32_t)(w0 & 0xFF);
             const uint32_t flags = (uint32_t)((w0 >> 16) & 0xFF);
@@ -328,25 +329,8 @@ struct LaneInterp {
             const uint32_t a2 = (uint32_t)(w0 >> 48);
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
-            if (op == kLaneFrameRun) {
-                // A packed run of frame opcodes.  The skip over the run's
-                // instructions is taken by every lane (pc stays uniform);
-                // the frame updates only by live lanes.
-                if (live) {
-                    for (uint32_t j = 0; j < a1; ++j) {
-                        const uint64_t w = j < 4 ? w1 : (j < 8 ? w2 : w3);
-                        const uint32_t f = (uint32_t)(w >> (16 * (j & 3u)));
-                        const uint32_t fop = f & 7u, fa = (f >> 3) & 63u, fb = (f >> 9) & 63u;
-                        if (fop == CLIFFT_OP_FRAME_CNOT) frame_cnot(fa, fb);
-                        else if (fop == CLIFFT_OP_FRAME_CZ) frame_cz(fa, fb);
-                        else if (fop == CLIFFT_OP_FRAME_SWAP) frame_swap(fa, fb);
-                        else if (fop == CLIFFT_OP_FRAME_H) frame_h(fa);
-                        else frame_s(fa);
-                    }
-                }
-                pc += a2;
-                continue;
-            }
+            // decided before the per-lane `live` test, so that pc stays uniform
+            skip = op >= kLaneFrameRun ? a2 : 0u;
             if (!live) continue;
             if (kFrameFirst && op <= CLIFFT_OP_FRAME_SWAP) {
                 if (op == CLIFFT_OP_FRAME_CNOT) frame_cnot(a1, a2);
@@ -357,6 +341,32 @@ struct LaneInterp {
                 continue;
             }
             switch (op) {
+                case kLaneFrameRun:  // a packed run of frame opcodes
+                    for (uint32_t j = 0; j < a1; ++j) {
+                        const uint64_t w = j < 4 ? w1 : (j < 8 ? w2 : w3);
+                        const uint32_t f = (uint32_t)(w >> (16 * (j & 3u)));
+                        const uint32_t fop = f & 7u, fa = (f >> 3) & 63u, fb = (f >> 9) & 63u;
+                        if (fop == CLIFFT_OP_FRAME_CNOT) frame_cnot(fa, fb);
+                        else if (fop == CLIFFT_OP_FRAME_CZ) frame_cz(fa, fb);
+                        else if (fop == CLIFFT_OP_FRAME_SWAP) frame_swap(fa, fb);
+                        else if (fop == CLIFFT_OP_FRAME_H) frame_h(fa);
+                        else frame_s(fa);
+                    }
+                    break;
+                case kLaneMeasPauli: {  // dormant measurement, then the APPLY_PAULI on its record
+                    uint32_t rec;
+                    if (((w0 >> 8) & 0xFF) == CLIFFT_OP_MEAS_DORMANT_STATIC) {
+                        rec = (flags & CLIFFT_FLAG_IDENTITY) ? sign : (gx(a1) ^ sign);
+                    } else {
+                        const uint32_t m = rng_double(rng) < 0.5 ? 0u : 1u;
+                        set_x(a1, m);
+                        set_z(a1, 0);
+                        rec = m ^ sign;
+                    }
+                    rset(rm, i0, rec);
+                    if (rec) apply_pauli(w2, w3);
+                    break;
+                }
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
                 case CLIFFT_OP_FRAME_CZ: frame_cz(a1, a2); break;
                 case CLIFFT_OP_FRAME_H: frame_h(a1); break;
