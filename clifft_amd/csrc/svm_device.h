@@ -176,6 +176,11 @@ constexpr uint64_t kMonoOps =
 constexpr uint64_t kMonoKeepOps =
     kMonoOps | (kNoAmpOps & ~((1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) |
                               (1ull << CLIFFT_OP_EXPAND_T_DAG) | (1ull << CLIFFT_OP_EXPAND_ROT)));
+// Active measurements, which apply a deferred run in their probability pass.
+constexpr uint64_t kMonoMeasOps =
+    (1ull << CLIFFT_OP_MEAS_ACTIVE_DIAGONAL) | (1ull << CLIFFT_OP_MEAS_ACTIVE_INTERFERE) |
+    (1ull << CLIFFT_OP_SWAP_MEAS_INTERFERE) | (1ull << CLIFFT_OP_MEAS_ACTIVE_DIAGONAL_FORCED) |
+    (1ull << CLIFFT_OP_MEAS_ACTIVE_INTERFERE_FORCED) | (1ull << CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED);
 
 // One deferred permutation/phase opcode of a fused run (global tier): the
 // run's opcodes are applied to every amplitude in one out-of-place pass.

@@ -177,6 +177,22 @@ struct Interp {
         nmono = 0;
     }
 
+    // Outcome probabilities of an active measurement.  A deferred run of two
+    // or more permutation/phase opcodes is applied in the same pass (into the
+    // other slice) instead of a gather of its own followed by a reading pass.
+    CSVM_INLINE double2 probs(uint32_t kind, uint32_t fa) {
+        if (nmono >= 2) {
+            team_sync<BT>();  // the list written by lane 0 is visible to every wave
+            const double2 pr = sweep_mono_probs<BT>(amp, amp_alt, k, mops, nmono, kind, fa, g, red);
+            AmpT* t = amp;
+            amp = amp_alt;
+            amp_alt = t;
+            nmono = 0;
+            return pr;
+        }
+        return sweep_probs<BT>(amp, k, kind, fa, g, red);
+    }
+
     // svm.h:177-186
     CSVM_INLINE void scale_magnitude(double s) {
         g_re *= s;
@@ -340,7 +356,7 @@ struct Interp {
     // :1573-1627; svm_forced_kernels.cc:118-171
     CSVM_INLINE bool meas_diagonal(uint32_t v, uint32_t idx, uint32_t sign, bool is_forced) {
         uint32_t px = f.gx(v), pz = f.gz(v);
-        double2 pr = sweep_probs<BT>(amp, k, PROB_DIAG, 0u, g, red);
+        double2 pr = probs(PROB_DIAG, 0u);
         double total = pr.x + pr.y;
         uint32_t b, m_abs, rec;
         if (!choose(pr.x, pr.y, total, px, sign, idx, is_forced, b, m_abs, rec)) return false;
@@ -357,7 +373,7 @@ struct Interp {
     // :1634-1708; svm_forced_kernels.cc:173-229
     CSVM_INLINE bool meas_interfere(uint32_t v, uint32_t idx, uint32_t sign, bool is_forced) {
         uint32_t px = f.gx(v), pz = f.gz(v);
-        double2 pr = sweep_probs<BT>(amp, k, PROB_INTERFERE, 0u, g, red);
+        double2 pr = probs(PROB_INTERFERE, 0u);
         double total = pr.x + pr.y;
         uint32_t bx, m_abs, rec;
         if (!choose(pr.x, pr.y, total, pz, sign, idx, is_forced, bx, m_abs, rec)) return false;
@@ -376,7 +392,7 @@ struct Interp {
         if (fa == t) return meas_interfere(t, idx, sign, is_forced);
         frame_swap(fa, t);
         uint32_t px = f.gx(t), pz = f.gz(t);
-        double2 pr = sweep_probs<BT>(amp, k, PROB_SWAP_INTERFERE, fa, g, red);
+        double2 pr = probs(PROB_SWAP_INTERFERE, fa);
         double total = pr.x + pr.y;
         uint32_t bx, m_abs, rec;
         if (!choose(pr.x, pr.y, total, pz, sign, idx, is_forced, bx, m_abs, rec)) return false;
@@ -541,7 +557,9 @@ struct Interp {
             const uint32_t a2 = (uint32_t)(w0 >> 48);
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
-            if (nmono && !((kMonoKeepOps >> op) & 1ull)) flush_mono();
+            // an active measurement takes a run of two or more itself (probs)
+            if (nmono && !((kMonoKeepOps >> op) & 1ull) && !(nmono >= 2 && ((kMonoMeasOps >> op) & 1ull)))
+                flush_mono();
             if (npend && !((kNoAmpOps >> op) & 1ull)) flush_expand();
             switch (op) {
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;

@@ -336,6 +336,81 @@ CSVM_SWEEP void sweep_mono(const AmpT* src, AmpT* dst, uint32_t k, const lds_mon
     team_sync<BT>();
 }
 
+// Pair layouts of the active measurements' probability sums.
+enum : uint32_t { PROB_DIAG = 0, PROB_INTERFERE = 1, PROB_SWAP_INTERFERE = 2 };
+
+// A deferred permutation/phase run and the outcome probabilities of the
+// active measurement that follows it, in one pass.  dst receives the run's
+// result, entry for entry as sweep_mono writes it; the sums are sweep_probs'
+// over that result, each lane visiting the same pair indices in the same
+// order, so the measurement reads the state once instead of twice.
+template <bool BT, class AmpT>
+CSVM_SWEEP double2 sweep_mono_probs(const AmpT* src, AmpT* dst, uint32_t k, const lds_mono* ops, uint32_t n,
+                                    uint32_t kind, uint32_t fa, TeamGeom g, lds_f64* red) {
+    k = uni(k); n = uni(n); kind = uni(kind); fa = uni(fa); g = uni_geom(g);
+    const uint32_t half = 1u << (k - 1), t = k - 1, fb = 1u << fa;
+    // pairs in flight per lane: as many loads as sweep_mono keeps
+    constexpr uint32_t U = Unroll<AmpT>::value > 1 ? Unroll<AmpT>::value / 2 : 1;
+    double v[2] = {0.0, 0.0};
+    for (uint32_t i0 = g.lane; i0 < half; i0 += U * g.size) {
+        uint32_t y[2 * U], x[2 * U];
+        dv2 a[2 * U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            // the pair sweep_probs reads for index i (in range or not,
+            // y stays below 2^k)
+            const uint32_t i = (i0 + u * g.size) & (half - 1u);
+            if (kind == PROB_SWAP_INTERFERE) {
+                const uint32_t bf = (i >> fa) & 1u;
+                y[2 * u] = (i & ~fb) | (bf << t);
+                y[2 * u + 1] = y[2 * u] | fb;
+            } else {
+                y[2 * u] = i;
+                y[2 * u + 1] = i + half;
+            }
+            x[2 * u] = y[2 * u];
+            x[2 * u + 1] = y[2 * u + 1];
+        }
+        for (uint32_t j = n; j-- > 0;) {
+            const MonoOp o = load_mono(ops + j);
+#pragma unroll
+            for (uint32_t e = 0; e < 2 * U; ++e) x[e] = mono_perm(o, x[e]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < half) {
+                a[2 * u] = src[x[2 * u]];
+                a[2 * u + 1] = src[x[2 * u + 1]];
+            }
+        for (uint32_t j = 0; j < n; ++j) {
+            const MonoOp o = load_mono(ops + j);
+#pragma unroll
+            for (uint32_t e = 0; e < 2 * U; ++e) {
+                a[e] = mono_phase(o, x[e], a[e]);
+                x[e] = mono_perm(o, x[e]);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < half) {
+                const dv2 p = a[2 * u], q = a[2 * u + 1];
+                dst[y[2 * u]] = p;
+                dst[y[2 * u + 1]] = q;
+                if (kind == PROB_DIAG) {
+                    v[0] += p.x * p.x + p.y * p.y;
+                    v[1] += q.x * q.x + q.y * q.y;
+                } else {
+                    const dv2 s = cadd(p, q), d = csub(p, q);
+                    v[0] += s.x * s.x + s.y * s.y;
+                    v[1] += d.x * d.x + d.y * d.y;
+                }
+            }
+    }
+    team_sync<BT>();
+    team_reduce<BT, 2>(v, g, red);
+    return make_double2(v[0], v[1]);
+}
+
 // EXPAND: amp[i + 2^k] = amp[i] * ph for i < 2^k (:876-945)
 template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_expand(AmpT* amp, uint32_t k, double pr, double pi, uint32_t has_phase, TeamGeom g) {
@@ -407,7 +482,6 @@ CSVM_SWEEP void sweep_scale(AmpT* amp, uint32_t k, double s, TeamGeom g) {
 // --------------------------------------------------------------------------
 // Measurement reductions and compaction (:1573-1804)
 // --------------------------------------------------------------------------
-enum : uint32_t { PROB_DIAG = 0, PROB_INTERFERE = 1, PROB_SWAP_INTERFERE = 2 };
 
 // Returns (p0, p1) of the branch pair on axis k-1 (or the swapped mapping
 // for SWAP_MEAS_INTERFERE with from-axis fa).
