@@ -103,6 +103,7 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
         S.f.init(wblock);
         S.kf = 0;
         S.npend = 0;
+        S.nmono = 0;  // a shot that ended early leaves no deferred run to the next one
         S.pend_mask = 0;
         S.dust = 0;
         S.discarded = 0;
