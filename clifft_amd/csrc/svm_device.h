@@ -59,6 +59,10 @@ constexpr uint32_t kLaneFrameRunMax = 12;
 // Lane superinstructions are the opcodes from kLaneFrameRun up; each names
 // in axis_2 how many instructions after it it covers.
 constexpr uint32_t kLaneMeasPauli = 0xF1;
+// Lane-bytecode opcode of a run of 2 or more DETECTORs whose parities are
+// all inline: the first instruction of the run, axis_2 = run length - 1; the
+// handler reads each detector's own instruction words.
+constexpr uint32_t kLaneDetectorRun = 0xF2;
 
 struct DevProgram {
     const clifft_instruction* bc;

@@ -400,6 +400,21 @@ void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code
         std::memcpy(in.p.raw + 8, &w2, 8);
         std::memcpy(in.p.raw + 16, &w3, 8);
     }
+    // runs of DETECTORs with inline parities: one dispatch for the run
+    auto inline_det = [&](uint64_t i) {
+        uint16_t marker;
+        std::memcpy(&marker, code[i].p.raw + 8, 2);
+        return code[i].opcode == CLIFFT_OP_DETECTOR && marker == kLaneInlineParity;
+    };
+    for (uint64_t i = 0; i < n;) {
+        uint64_t j = i;
+        while (j < n && j - i < 0xFFFF && inline_det(j)) ++j;
+        if (j - i >= 2) {
+            code[i].opcode = (uint8_t)kLaneDetectorRun;
+            code[i].axis_2 = (uint16_t)(j - i - 1);
+        }
+        i = std::max(j, i + 1);
+    }
 }
 
 // A dormant measurement immediately followed by the APPLY_PAULI that its

@@ -353,6 +353,14 @@ struct LaneInterp {
                         else frame_s(fa);
                     }
                     break;
+                case kLaneDetectorRun:  // DETECTORs with inline parities, each from its own words
+                    for (uint32_t j = 0; j <= a2; ++j) {
+                        const uint64_t x0 = code[4 * (pc + j)], x1 = code[4 * (pc + j) + 1];
+                        const uint64_t x2 = code[4 * (pc + j) + 2], x3 = code[4 * (pc + j) + 3];
+                        const uint32_t e = (((uint32_t)(x0 >> 16) & 0xFF) & CLIFFT_FLAG_EXPECTED_ONE) ? 1u : 0u;
+                        rset(rd, (uint32_t)(x1 >> 32), e ^ inline_parity(x2, x3));
+                    }
+                    break;
                 case kLaneMeasPauli: {  // dormant measurement, then the APPLY_PAULI on its record
                     uint32_t rec;
                     if (((w0 >> 8) & 0xFF) == CLIFFT_OP_MEAS_DORMANT_STATIC) {
