@@ -1301,6 +1301,17 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
 // ---------------------------------------------------------------------------
 // Batched persistent executor
 // ---------------------------------------------------------------------------
+// The persistent multi-program launch is switched off: its first device run
+// ended in a GPU memory fault whose cause has not been found (a CPU
+// emulation of the same programs under AddressSanitizer found no
+// out-of-bounds access).  Until a cause is shown, every entry point that
+// would launch it returns CLIFFT_E_UNSUPPORTED; clifft_svm_batch_sample with
+// CLIFFT_BATCH_PER_PROGRAM runs the same programs with one plain launch each.
+constexpr bool kPersistentBatchEnabled = false;
+constexpr const char* kPersistentBatchOff =
+    "the persistent batch launch is switched off in this build (an unexplained GPU memory fault); "
+    "use CLIFFT_BATCH_PER_PROGRAM";
+
 struct clifft_svm_batch {
     std::vector<clifft_svm_program*> progs;
     std::vector<uint64_t> shots;
@@ -1621,6 +1632,7 @@ int clifft_svm_batch_get_tuning(const clifft_svm_batch* b, clifft_svm_batch_tuni
 
 int clifft_svm_batch_run(clifft_svm_batch* b, const uint64_t* seeds, void* stream) {
     if (!b) return fail(CLIFFT_E_INVALID, "null batch");
+    if (!kPersistentBatchEnabled) return fail(CLIFFT_E_UNSUPPORTED, kPersistentBatchOff);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     int rc = batch_upload_seeds(b, seeds, st);
     if (rc) return rc;
@@ -1631,6 +1643,7 @@ int clifft_svm_batch_run(clifft_svm_batch* b, const uint64_t* seeds, void* strea
 
 int clifft_svm_batch_run_graph(clifft_svm_batch* b, const uint64_t* seeds, void* stream) {
     if (!b) return fail(CLIFFT_E_INVALID, "null batch");
+    if (!kPersistentBatchEnabled) return fail(CLIFFT_E_UNSUPPORTED, kPersistentBatchOff);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     int rc = batch_upload_seeds(b, seeds, st);
     if (rc) return rc;
@@ -1663,6 +1676,7 @@ int clifft_svm_batch_run_graph(clifft_svm_batch* b, const uint64_t* seeds, void*
 
 int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stream, clifft_svm_batch_tuning* chosen) {
     if (!b) return fail(CLIFFT_E_INVALID, "null batch");
+    if (!kPersistentBatchEnabled) return fail(CLIFFT_E_UNSUPPORTED, kPersistentBatchOff);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     if (sample_shots == 0) sample_shots = ~0ull;
     std::vector<uint64_t> seeds(b->progs.size());

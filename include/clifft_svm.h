@@ -388,7 +388,12 @@ int clifft_svm_batch_create(clifft_svm_program* const* programs, const uint64_t*
 int clifft_svm_batch_destroy(clifft_svm_batch* batch);
 int clifft_svm_batch_set_tuning(clifft_svm_batch* batch, const clifft_svm_batch_tuning* tuning);
 int clifft_svm_batch_get_tuning(const clifft_svm_batch* batch, clifft_svm_batch_tuning* tuning);
-/* Autotuner: times candidate tunings on the batch itself (each program
+/* In this build the persistent launch is switched off: clifft_svm_batch_tune,
+ * clifft_svm_batch_run, clifft_svm_batch_run_graph and clifft_svm_batch_sample
+ * without CLIFFT_BATCH_PER_PROGRAM return CLIFFT_E_UNSUPPORTED (its first device
+ * run ended in a GPU memory fault whose cause has not been found).
+ *
+ * Autotuner: times candidate tunings on the batch itself (each program
  * capped at `sample_shots` shots, outputs overwritten) with HIP events on
  * `stream`, keeps the fastest and returns it in `chosen` (may be NULL). */
 int clifft_svm_batch_tune(clifft_svm_batch* batch, uint64_t sample_shots, void* stream,
