@@ -313,17 +313,29 @@ CSVM_INLINE uint32_t upper_bound_in(PD hz, uint32_t lo, uint32_t hi, double targ
 // std::upper_bound(hz, hz + n, target) for n >= 1 (the gap sampler's next
 // firing site).  The guide holds, for G buckets of width H / G over
 // [0, H = hz[n-1]), the upper bound of each bucket's lower edge
-// (guide[G] = n).  The search runs over the sites within one bucket either
-// side of the target's bucket; the answer is checked against its two
-// neighbours and, should the bucket index have rounded past the window,
-// replaced by a search of the whole table.  The result is upper_bound's in
-// every case.
+// (guide[G] = n).  First try: that bound or the site after it, each
+// accepted only if its neighbours bracket the target (with G about the site
+// count most buckets hold one site).  Then the sites within one bucket
+// either side of the target's bucket, the answer checked against its
+// neighbours; should the bucket index have rounded past that window, a
+// search of the whole table.  The result is upper_bound's in every case.
 template <class PD, class PU>
 CSVM_INLINE uint32_t hazard_upper_bound(PD hz, uint32_t n, PU guide, uint32_t G, double scale, double target) {
     if (!(target < hz[n - 1])) return n;
     if (G == 0) return upper_bound_in(hz, 0, n, target);
     const double fb = target * scale;
     const uint32_t b = fb < (double)G ? (uint32_t)fb : G - 1u;
+    {
+        const uint32_t g0 = guide[b];
+        // three loads at once, indices clamped into the table (hazards are
+        // >= 0, so -1 stands for "no site before", +inf for "none after")
+        const double hp = hz[g0 > 0 ? g0 - 1u : 0u], h0 = hz[g0 < n ? g0 : n - 1u],
+                     h1 = hz[g0 + 1u < n ? g0 + 1u : n - 1u];
+        if ((g0 == 0 ? -1.0 : hp) <= target) {
+            if (g0 >= n || h0 > target) return g0;
+            if (g0 + 1u >= n || h1 > target) return g0 + 1u;
+        }
+    }
     uint32_t lo = upper_bound_in(hz, guide[b > 0 ? b - 1u : 0u], guide[b + 2u < G ? b + 2u : G], target);
     if ((lo > 0 && hz[lo - 1] > target) || (lo < n && hz[lo] <= target)) lo = upper_bound_in(hz, 0, n, target);
     return lo;
@@ -331,9 +343,17 @@ CSVM_INLINE uint32_t hazard_upper_bound(PD hz, uint32_t n, PU guide, uint32_t G,
 
 // First channel c in [c0, c1) with r < cum[c], c1 if none: the reference's
 // running-sum scan, on running sums precomputed in the same order (so the
-// same doubles); a binary search, as the sums never decrease.
+// same doubles).  r = u * cum[c1 - 1]; the channel u would pick if the
+// channels were equally likely is tried first (accepted only if its running
+// sums bracket r), then a binary search, as the sums never decrease.
 template <class PD>
-CSVM_INLINE uint32_t pick_channel(PD cum, uint32_t c0, uint32_t c1, double r) {
+CSVM_INLINE uint32_t pick_channel(PD cum, uint32_t c0, uint32_t c1, double r, double u) {
+    {
+        const uint32_t cnt = c1 - c0, q = (uint32_t)(u * (double)cnt);
+        const uint32_t g = c0 + (q < cnt ? q : cnt - 1u);
+        const double below = cum[g > c0 ? g - 1u : g], at = cum[g];
+        if ((g == c0 || below <= r) && r < at) return g;
+    }
     uint32_t lo = c0, hi = c1;
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;

@@ -309,7 +309,7 @@ struct Interp {
         const uint32_t c0 = off[site], c1 = off[site + 1];
         const double u = rng_double(rng);
         if (c1 > c0) {
-            const uint32_t c = uni(pick_channel(cum, c0, c1, u * cum[c1 - 1]));
+            const uint32_t c = uni(pick_channel(cum, c0, c1, u * cum[c1 - 1], u));
             if (c < c1) {
                 uint32_t W = P->words;
                 apply_pauli(cp(P->chan_x) + (size_t)c * W, cp(P->chan_z) + (size_t)c * W, W, 0);

@@ -166,7 +166,7 @@ struct LaneInterp {
         const double u = rng_double(rng);
         if (c1 > c0) {
             const double* cum = P->chan_cum;
-            const uint32_t c = pick_channel(cum, c0, c1, u * cum[c1 - 1]);
+            const uint32_t c = pick_channel(cum, c0, c1, u * cum[c1 - 1], u);
             if (c < c1) apply_pauli(P->chan_x[c], P->chan_z[c]);
         }
         next_noise++;
