@@ -330,15 +330,18 @@ size_t rec_stride_of(const clifft_svm_program* p, bool kfault = false) {
            (kfault ? 20u * kMaxForcedFaults : 0u);
 }
 
-// Operands the lane interpreter reads from the instruction itself (single
-// frame word programs), written into payload bytes 8..23 that these opcodes
-// do not use:
+// The lane interpreter's copy of the bytecode (single frame word programs).
+// Operands it reads from the instruction itself go into payload bytes 8..23,
+// which these opcodes do not use:
 //   APPLY_PAULI: the X and Z masks of the frame correction;
 //   DETECTOR, POSTSELECT, OBSERVABLE: the record parity as (word, mask)
 //     pairs over the 32-bit words of the measurement record -- bytes 8..9
 //     hold kLaneInlineParity, 10..11 word a, 12..15 mask a, 16..17 word b,
 //     20..23 mask b.  Targets listed twice cancel, as in the XOR they feed.
 //     Lists spanning more than two words keep the CSR tables (marker 0).
+// Runs of frame opcodes become kLaneFrameRun and runs of inline-parity
+// DETECTORs kLaneDetectorRun (svm_device.h); the instructions they cover
+// stay in place.
 void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code) {
     // runs of frame opcodes become one packed instruction (first of the run)
     const uint64_t n = v->num_instructions;
@@ -386,8 +389,8 @@ void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code
             }
             words.erase(std::remove_if(words.begin(), words.end(), [](const auto& e) { return e.second == 0; }),
                         words.end());
-            if (words.size() > 2 || (words.size() > 0 && words.back().first > 0xFFFF) ||
-                (words.size() > 1 && words[0].first > 0xFFFF))
+            if (words.size() > 2 ||
+                std::any_of(words.begin(), words.end(), [](const auto& e) { return e.first > 0xFFFF; }))
                 continue;  // marker stays 0: CSR tables
             uint32_t wa = 0, ma = 0, wb = 0, mb = 0;
             if (words.size() > 0) wa = words[0].first, ma = words[0].second;

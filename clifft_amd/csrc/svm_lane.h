@@ -302,8 +302,11 @@ struct LaneInterp {
     }
 
     // ------------------------------------------------------------- dispatch
-    // Runs the whole bytecode for this lane's shot; returns early (for this
-    // lane only) on post-selection discard or an unreachable forced outcome.
+    // Runs this lane's shot over the lane copy of the bytecode (operands
+    // inlined, superinstructions for runs of frame opcodes, detector runs and
+    // measurement + reset pairs; see DevProgram::lane_bc).  A shot that ends
+    // early (post-selection discard, unreachable forced outcome) stops
+    // executing opcodes but stays in the loop, so pc remains uniform.
     // kFrameFirst: dispatch the frame opcodes (0..5) before the general
     // switch.  Measured on one MI355X: 11% faster on the 20-qubit Clifford+T
     // (peak 6, the 4-workgroup build), 2% slower on the surface-code memory
