@@ -10,7 +10,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from clifft_amd.program import (Op, U2_DTYPE, U4_DTYPE, build_program, make_apply_pauli, make_array_cnot,
+from clifft_amd.program import (FLAG_IDENTITY, Op, U2_DTYPE, U4_DTYPE, build_program, make_apply_pauli,
+                                make_array_cnot,
                                 make_array_cz, make_array_h, make_array_multi_cnot, make_array_multi_cz,
                                 make_array_rot, make_array_s, make_array_s_dag, make_array_swap,
                                 make_array_t, make_array_t_dag, make_array_u2, make_array_u4,
@@ -175,7 +176,7 @@ def random_program(seed, num_qubits=10, peak=6, length=200, noise=True, postsele
             ax = int(rng.integers(k, n))
             sign = bool(rng.random() < 0.3)
             if rng.random() < 0.5:
-                flags = 4 if rng.random() < 0.2 else 0
+                flags = FLAG_IDENTITY if rng.random() < 0.2 else 0  # record = sign bit
                 bc.append(make_meas(Op.MEAS_DORMANT_STATIC, ax, slot, sign, flags=flags))
             else:
                 bc.append(make_meas(Op.MEAS_DORMANT_RANDOM, ax, slot, sign))
@@ -183,7 +184,9 @@ def random_program(seed, num_qubits=10, peak=6, length=200, noise=True, postsele
         elif r < 0.90 and noise:
             nch = int(rng.choice([1, 3, 15]))
             p = 10 ** rng.uniform(-3, -0.5)
-            site = [(*random_pauli_bits(rng, n), p / nch) for _ in range(nch)]
+            # equal channel probabilities (DEPOLARIZE-like) or unequal ones
+            w = np.full(nch, 1.0 / nch) if rng.random() < 0.5 else rng.dirichlet(np.ones(nch))
+            site = [(*random_pauli_bits(rng, n), float(p * w[c])) for c in range(nch)]
             if rng.random() < 0.3:  # a block of consecutive sites
                 cnt = int(rng.integers(1, 4))
                 start = len(sites)

@@ -34,16 +34,21 @@ def svm():
     return svm
 
 
-# (num_qubits, peak): one shot per lane (peak <= 3, <= 64 qubits), wave-team
-# LDS, workgroup-team LDS, global-memory tier, and >64-qubit frames
+# (num_qubits, peak): one shot per lane (peak <= 6, <= 64 qubits), wave-team
+# LDS, workgroup-team LDS, global-memory tier up to 2^20 amplitudes (16 MiB
+# states), and >64-qubit frames
 SHAPES = [(5, 0), (6, 1), (10, 2), (8, 3), (64, 3), (12, 6), (16, 8), (20, 10), (24, 12), (24, 13), (24, 14),
-          (30, 15), (100, 3), (100, 6), (130, 9)]
+          (30, 15), (24, 16), (26, 18), (28, 20), (100, 3), (100, 6), (130, 9)]
+
+
+def shots_for(peak):
+    return 300 if peak < 12 else (64 if peak <= 16 else (8 if peak <= 18 else 3))
 
 
 @pytest.mark.parametrize("nq,peak", SHAPES)
 @pytest.mark.parametrize("seed", [1, 2])
 def test_sample_bit_exact(svm, orc, nq, peak, seed):
-    shots = 64 if peak >= 12 else 300
+    shots = shots_for(peak)
     announce("sample", nq, peak, seed, shots)
     p = random_program(1000 * seed + peak, num_qubits=nq, peak=peak, length=160 + 10 * peak)
     res = svm.sample(p, shots, seed=seed)
@@ -168,17 +173,73 @@ def test_fanout_cnot_runs(svm, orc, peak):
         np.testing.assert_allclose(res.exp_vals, e, atol=ATOL, rtol=0)
 
 
-@pytest.mark.parametrize("peak,shots,seed", [(14, 1100, 4263), (15, 600, 4257)])
+def mono_ops_at_rank(p, kmin):
+    """Permutation/phase opcodes executed at static rank >= kmin (the ones
+    the global tier fuses into runs)."""
+    from clifft_amd.program import Op
+    mono = {Op.ARRAY_CNOT, Op.ARRAY_CZ, Op.ARRAY_SWAP, Op.ARRAY_MULTI_CNOT, Op.ARRAY_MULTI_CZ, Op.ARRAY_S,
+            Op.ARRAY_S_DAG, Op.ARRAY_T, Op.ARRAY_T_DAG, Op.ARRAY_ROT}
+    phase = {Op.ARRAY_T, Op.ARRAY_T_DAG, Op.ARRAY_ROT}
+    h = p.active_k_history()
+    ops, a1 = p.bytecode["opcode"], p.bytecode["axis_1"]
+    return sum(1 for i, op in enumerate(ops) if op in mono and h[i] >= kmin and (op not in phase or a1[i] < h[i]))
+
+
+@pytest.mark.parametrize("peak,shots,seed", [(14, 1100, 4270), (15, 600, 4202)])
 def test_global_tier_team_sizes(svm, orc, peak, shots, seed):
     """Enough shots that the launch picks smaller global-memory teams (256
     threads at peak 14, 512 at peak 15); the programs reach their peak rank
-    and hold 23 and 27 permutation/phase opcodes at rank >= 10 (fused runs):
-    records bit-exact and expectation values within 1e-12 of the oracle."""
+    and hold more than 50 permutation/phase opcodes at rank >= 10 (fused
+    runs): records bit-exact and expectation values within 1e-12 of the
+    oracle."""
     announce("global-teams", peak, shots, seed)
     p = random_program(seed, num_qubits=peak + 6, peak=peak, length=220, p_array=0.5)
+    assert p.active_k_history().max() == peak and mono_ops_at_rank(p, 10) > 50
     r = svm.sample(p, shots, seed=99)
     m, d, o, e, _ = orc.sample(p, shots, seed=99, per_shot=True)
     assert np.array_equal(r.measurements, m)
     assert np.array_equal(r.detectors, d)
     assert np.array_equal(r.observables, o)
     np.testing.assert_allclose(r.exp_vals, e, atol=ATOL, rtol=0)
+
+
+def identity_and_unequal(p):
+    """Whether a program holds a FLAG_IDENTITY measurement and a noise site
+    whose channels have unequal probabilities."""
+    from clifft_amd.program import FLAG_IDENTITY, Op
+    ident = bool(((p.bytecode["opcode"] == Op.MEAS_DORMANT_STATIC) & ((p.bytecode["flags"] & FLAG_IDENTITY) != 0)).any())
+    off, prob = p.site_chan_off, p.chan_prob
+    unequal = any(len(set(prob[off[s]:off[s + 1]].tolist())) > 1 for s in range(len(off) - 1))
+    return ident, unequal
+
+
+@pytest.mark.parametrize("nq,peak", [(6, 1), (10, 4), (12, 6), (14, 7), (16, 9), (20, 12), (100, 5)])
+def test_identity_meas_and_unequal_channels(svm, orc, nq, peak):
+    """Programs holding FLAG_IDENTITY measurements (record = sign bit,
+    svm_kernels.inl:2361-2369) and noise sites with unequal channel
+    probabilities (channel chosen by the running sum, :1899-1911) on every
+    tier: records bit-exact against the oracle, sampled and forced."""
+    announce("identity-unequal", nq, peak)
+    hits = [False, False]
+    for seed in range(4):
+        p = random_program(5000 + 17 * seed + peak, num_qubits=nq, peak=peak, length=300)
+        ident, unequal = identity_and_unequal(p)
+        hits[0] |= ident
+        hits[1] |= unequal
+        res = svm.sample(p, 500, seed=seed)
+        m, d, o, e, _ = orc.sample(p, 500, seed=seed, per_shot=True)
+        assert np.array_equal(res.measurements, m)
+        assert np.array_equal(res.detectors, d)
+        assert np.array_equal(res.observables, o)
+    assert hits == [True, True]
+    # forced execution of a measure-only program holding identity measurements
+    q = next(q for q in (random_measure_only_program(s, num_qubits=max(nq, 8), peak=min(peak, 8), length=150)
+                         for s in range(900 + peak, 960 + peak))
+             if identity_and_unequal(q)[0] and 0 < q.num_measurements <= 64)
+    recs, *_ = orc.sample(q, 30, seed=4)
+    recs[::4] ^= 1  # some unreachable, identity records among them
+    gpu = svm.record_probabilities(q, recs)
+    ref = orc.record_log_probs(q.forced(), recs)
+    assert np.array_equal(gpu < -1e300, ref < -1e300)
+    ok = ref > -1e300
+    np.testing.assert_allclose(gpu[ok], ref[ok], atol=1e-10, rtol=1e-12)
