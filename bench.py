@@ -2,18 +2,27 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A step is one sampling pass over a batch of shots of the workload's compiled
-program, with the bit-packed records written to device memory (inputs and
-outputs resident in HBM).  The default workload is the noisy distance-3
-surface-code memory circuit (1e6 shots per step).  Prints one JSON line.
+A step is one sampling pass over the workload's shots, with the bit-packed
+records written to device memory (inputs and outputs resident in HBM).
+Prints one JSON line.
 
-roofline: algorithmic bytes of one launch (per-shot state-traffic model:
-16 B per amplitude read or written by each array opcode at its static rank,
-plus the record bytes written) / the shot kernel's average duration from HIP
-events on its stream, against the 8 TB/s HBM peak.  `traffic` is the HBM
-bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters
-(profiles/hbm_traffic.json, scaled by a copy-kernel calibration), or null
-when no measurement exists for the workload at this shot count.
+Default workload (batch4096): 4096 random Clifford+T programs (16-40
+qubits, peak rank 4-16) x 1000 shots, run as a batch -- one persistent
+launch per launch group, the groups side by side -- against one plain launch
+per program on the same stream ("speedup_vs_per_program").  The other
+workloads time one program's sampling launch (surface_d3, clifford_t20,
+large_k34, clifford_t24_k16), the per-program launches alone (programs4096)
+or the adjoint gradient (rz24_grad).
+
+roofline: algorithmic bytes of the dominant kernel's launch (per-shot
+state-traffic model: 16 B per amplitude read or written by each array opcode
+at its static rank, plus the record bytes written) / that kernel's duration
+from HIP events on the stream it runs on, against the 8 TB/s HBM peak; when
+fused passes move fewer bytes than the model charges, achieved can pass the
+peak and "effective" is set.  `traffic` is the HBM bytes per launch measured
+with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (profiles/hbm_traffic.json,
+scaled by a copy-kernel calibration), with traffic_frac = traffic / duration
+/ peak, or null when no measurement exists for the workload.
 cpu_baseline: the C oracle built with -O3 -ffast-math -fopenmp, single PRNG
 stream, shots in sequence (the reference's scheduling), on a bounded sample;
 the shot-parallel OpenMP figure on all host cores rides along as "mode_b".
@@ -43,10 +52,9 @@ WORKLOADS = {
     "rz24_grad": ("24 qubits, 30 Clifford layers, 16 R_Z(theta_i) slots (peak k=16), 32 random Pauli probes "
                   "with nonzero expectation; "
                   "<O> and d<O>/dtheta for 512 theta vectors by the adjoint kernel", 512),
-    "programs4096": ("the 4096 programs of batch4096, 1000 shots each, one plain launch per program "
-                     "(the persistent batch launch is kept off the device)", 1000),
+    "programs4096": ("the 4096 programs of batch4096, 1000 shots each, one plain launch per program", 1000),
     "batch4096": ("4096 random Clifford+T programs, n ~ U[16,40], T-count ~ U[4,16], 40 layers, final M; "
-                  "one persistent launch", 1000),
+                  "one persistent launch per launch group", 1000),
 }
 
 
@@ -224,9 +232,10 @@ def run_grad(args):
     print(json.dumps(line), flush=True)
 
 
-def _measure_batch(progs, shots, seeds0, steps, warmup):
-    """One persistent launch (autotuned) vs one plain launch per program, on
-    the same stream and outputs.  Returns timings in seconds."""
+def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots):
+    """The batch (autotuned on a `tune_shots`-per-program sample) against one
+    plain launch per program, on the same stream and outputs.  Returns
+    timings in seconds and the per-group kernel times of one eager run."""
     import ctypes
 
     import torch
@@ -244,8 +253,11 @@ def _measure_batch(progs, shots, seeds0, steps, warmup):
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
     bt = B.Batch(progs, [shots] * count, outs, packed=True)
-    tuning = bt.tune(sample_shots=max(1, shots // 8), stream=sh)
+    t_tune = time.perf_counter()
+    tuning = bt.tune(sample_shots=tune_shots, stream=sh)
     torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t_tune
+    groups = bt.groups()
     recs = (_lib.Records * count)(*[_lib.Records(o["measurements"], None, None, None, None) for o in outs])
     harr = (ctypes.c_void_p * count)(*[h.handle.value for h in handles])
     shots_arr = np.full(count, shots, dtype=np.uint64)
@@ -263,12 +275,19 @@ def _measure_batch(progs, shots, seeds0, steps, warmup):
         per_program(seeds0 + np.uint64(r + 1))
     torch.cuda.synchronize()
     t_pp = (time.perf_counter() - t0) / reps_pp
+    # per-group kernel times (timing events on each group's stream)
+    bt.time_groups(True)
+    bt.run(seeds0 + np.uint64(99), stream=sh)
+    torch.cuda.synchronize()
+    group_ms = bt.group_times()
+    bt.time_groups(False)
     for i in range(warmup):
         bt.run(seeds0 + np.uint64(100 + i), stream=sh)
     torch.cuda.synchronize()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     with torch.cuda.stream(stream):
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(steps):
             starts[i].record(stream)
@@ -283,11 +302,43 @@ def _measure_batch(progs, shots, seeds0, steps, warmup):
             bt.run_graph(seeds0 + np.uint64(2000 + i), stream=sh)
         torch.cuda.synchronize()
         wall_graph = time.perf_counter() - t0
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-    alg = sum((h.info()["bytes_per_shot"] + nb(p.num_measurements)) * shots for h, p in zip(handles, progs))
+    step_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    per_prog_bytes = [(h.info()["bytes_per_shot"] + nb(p.num_measurements)) * shots for h, p in zip(handles, progs)]
     bt.close()
-    return {"t_batch": wall / steps, "t_graph": wall_graph / steps, "t_per_program": t_pp, "kern_ms": kern_ms,
-            "alg_bytes": alg, "tuning": tuning}
+    return {"t_batch": wall / steps, "t_graph": wall_graph / steps, "t_per_program": t_pp, "step_ms": step_ms,
+            "per_prog_bytes": per_prog_bytes, "tuning": tuning, "groups": groups, "group_ms": group_ms,
+            "tune_s": t_tune}
+
+
+def batch_cpu_baseline(progs, shots_total, run_seconds=12.0):
+    """The C oracle (-O3 -ffast-math -fopenmp) on a bounded sample of the
+    batch: every 64th program (the same peak-rank mix), with a per-program
+    shot count sized to about `run_seconds` of single-core work; mode A = one
+    stream per program, shots in sequence (the reference's scheduling), mode
+    B = OpenMP over shots on every host core."""
+    from oracle import oracle
+    sample = progs[::64]
+    t0 = time.perf_counter()
+    for i, p in enumerate(sample):
+        oracle.sample(p, 4, seed=7 + i, per_shot=False, fast=True)
+    per4 = time.perf_counter() - t0
+    n = int(max(4, min(shots_total, 4 * run_seconds / max(per4, 1e-9))))
+    t0 = time.perf_counter()
+    for i, p in enumerate(sample):
+        oracle.sample(p, n, seed=7 + i, per_shot=False, fast=True)
+    t_a = time.perf_counter() - t0
+    threads = oracle.lib(True).orc_max_threads()
+    nb_ = max(threads, n * max(1, threads // 2))
+    t0 = time.perf_counter()
+    for i, p in enumerate(sample):
+        oracle.sample_parallel(p, nb_, seed=7 + i, threads=threads, fast=True)
+    t_b = time.perf_counter() - t0
+    host = f"{os.cpu_count()} logical CPUs, {_cpu_model()}"
+    return {"value": len(sample) * n / t_a, "unit": "shots/s", "cores": 1, "kind": "port",
+            "sample": f"mode A: every 64th program of the batch ({len(sample)} programs, the same peak-rank mix), "
+                      f"{n} shots each, one stream per program, {t_a:.2f} s; host {host}",
+            "mode_b": {"value": len(sample) * nb_ / t_b, "unit": "shots/s", "cores": threads,
+                       "sample": f"{nb_} shots per sampled program, OpenMP over shots, {t_b:.2f} s"}}
 
 
 def run_programs(args):
@@ -349,52 +400,62 @@ def run_programs(args):
 
 
 def run_batch(args):
-    """Config: 4096 programs x 1000 shots in one persistent launch (autotuned),
-    against one plain launch per program on the same stream.  Also reported:
-    the same comparison on the programs of peak rank <= 13 alone (the part
-    that one plain launch per program cannot fill the chip with)."""
+    """Config: 4096 programs x 1000 shots (16-40 qubits, T-count = peak rank
+    4-16, 40 layers, final M on every qubit) as one batch: one persistent
+    launch per launch group, groups side by side, autotuned; against one
+    plain launch per program on the same stream and outputs."""
     from clifft_amd import _lib
     from clifft_amd import workloads as W
     from clifft_amd.compiler import compile as ccompile
 
-    raise SystemExit("batch4096: the persistent batch launch is switched off in this build (an unexplained GPU "
-                     "memory fault on its first device run); --workload programs4096 runs the same programs "
-                     "with one launch each")
     desc, shots = WORKLOADS["batch4096"]
     count = int(os.environ.get("CLIFFT_BENCH_BATCH_COUNT", "4096"))
     if args.shots:
         shots = args.shots
     progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
     seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
-    full = _measure_batch(progs, shots, seeds0, args.steps, args.warmup)
-    small = [i for i, p in enumerate(progs) if p.peak_rank <= 13]
-    sub = _measure_batch([progs[i] for i in small], shots, seeds0[small], args.steps, args.warmup)
+    r = _measure_batch(progs, shots, seeds0, args.steps, args.warmup, tune_shots=max(1, shots // 16))
     total = shots * count
-    value = total / full["t_batch"]
-    achieved = full["alg_bytes"] / (full["kern_ms"] * 1e-3) / 1e9
+    value = total / r["t_batch"]
+    # dominant kernel: the launch group with the longest kernel time
+    gi = int(np.argmax(r["group_ms"]))
+    g = r["groups"][gi]
+    dom_bytes = g["algorithmic_bytes"]
+    dom_ms = r["group_ms"][gi]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        w = tr["workloads"].get("batch4096_group_" + g["kind"])
+        if w and w["shots"] == shots and w.get("programs") == count:
+            traffic = (w["fetch_kib"] * tr["fetch_scale"] + w["write_kib"] * tr["write_scale"]) * 1024.0
     peaks = np.bincount([p.peak_rank for p in progs], minlength=17)
     line = {
         "metric": "shots/s", "value": value, "unit": "shots/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": full["t_batch"] * 1e3, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": r["t_batch"] * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batch4096", "description": desc, "programs": count, "shots_per_program": shots,
                    "peak_rank_histogram": peaks.tolist(),
                    "mean_instructions": float(np.mean([len(p.bytecode) for p in progs])),
-                   "tuning": full["tuning"], "records": "bit-packed measurements, device memory",
-                   "per_program_launches_shots_per_s": total / full["t_per_program"],
-                   "speedup_vs_per_program": full["t_per_program"] / full["t_batch"],
-                   "graph_replay_shots_per_s": total / full["t_graph"],
-                   "peak_le_13_subset": {"programs": len(small),
-                                         "batch_shots_per_s": shots * len(small) / sub["t_batch"],
-                                         "per_program_shots_per_s": shots * len(small) / sub["t_per_program"],
-                                         "speedup_vs_per_program": sub["t_per_program"] / sub["t_batch"],
-                                         "tuning": sub["tuning"]},
+                   "tuning": r["tuning"], "tune_seconds": r["tune_s"], "launch_groups": r["groups"],
+                   "group_kernel_ms": r["group_ms"], "records": "bit-packed measurements, device memory",
+                   "per_program_launches_shots_per_s": total / r["t_per_program"],
+                   "speedup_vs_per_program": r["t_per_program"] / r["t_batch"],
+                   "graph_replay_shots_per_s": total / r["t_graph"],
+                   "speedup_graph_vs_per_program": r["t_per_program"] / r["t_graph"],
                    "parallelism": "programs x shots"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": full["kern_ms"],
-                     "algorithmic_bytes_per_launch": full["alg_bytes"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"launch group {gi} ({g['kind']}, {g['programs']} programs, block {g['block']})",
+                     "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes,
+                     "traffic_frac": None if traffic is None else traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "effective": achieved > HBM_PEAK_GBS},
         "backend": _lib.backend(),
     }
+    if not args.no_cpu:
+        line["cpu_baseline"] = batch_cpu_baseline(progs, shots)
     print(json.dumps(line), flush=True)
 
 
@@ -403,7 +464,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="surface_d3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="batch4096", choices=sorted(WORKLOADS))
     ap.add_argument("--shots", type=int, default=0, help="override shots per step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",

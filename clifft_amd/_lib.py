@@ -54,10 +54,17 @@ class State(ctypes.Structure):
 
 
 class BatchTuning(ctypes.Structure):
-    _fields_ = [("block_threads", ctypes.c_uint32), ("wave_rank_max", ctypes.c_uint32),
+    _fields_ = [("global_threads", ctypes.c_uint32), ("wave_rank_max", ctypes.c_uint32),
                 ("lds_rank_max", ctypes.c_uint32), ("wave_chunk", ctypes.c_uint32),
-                ("block_chunk", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("block_chunk", ctypes.c_uint32), ("concurrent", ctypes.c_uint32),
                 ("lane_rank_max", ctypes.c_int32), ("lane_chunk", ctypes.c_uint32)]
+
+
+class BatchGroupInfo(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("programs", ctypes.c_uint32), ("items", ctypes.c_uint32),
+                ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32),
+                ("amp_elems", ctypes.c_uint32), ("amp_slices", ctypes.c_uint32),
+                ("algorithmic_bytes", ctypes.c_double)]
 
 
 # name -> (restype, argtypes); exactly the functions include/clifft_svm.h declares
@@ -89,6 +96,11 @@ SIGNATURES = {
     "clifft_svm_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "clifft_svm_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchTuning)]),
     "clifft_svm_batch_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchTuning)]),
+    "clifft_svm_batch_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(BatchGroupInfo),
+                                             ctypes.POINTER(ctypes.c_uint32)]),
+    "clifft_svm_batch_time_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "clifft_svm_batch_group_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                                    ctypes.POINTER(ctypes.c_uint32)]),
     "clifft_svm_batch_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
                                              ctypes.POINTER(BatchTuning)]),
     "clifft_svm_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -9,8 +9,8 @@ every program are bit-identical to `svm.sample(program, shots, seed)`.
         -> list[SampleResult]   (host outputs; per_program=True issues one
                                  plain launch per program instead)
     Batch(programs, shots, outs, packed=False)
-        device-output handle: tune(), tuning, set_tuning(), run(seeds),
-        run_graph(seeds)
+        device-output handle: tune(), tuning, set_tuning(), groups(),
+        run(seeds), run_graph(seeds)
 """
 
 from __future__ import annotations
@@ -101,6 +101,32 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def groups(self) -> list:
+        """The current plan's launch groups (kind: wave / lds / global / lane)."""
+        n = ctypes.c_uint32()
+        _lib.check(_lib.lib().clifft_svm_batch_info(self.handle, 0, None, ctypes.byref(n)), "batch_info")
+        arr = (_lib.BatchGroupInfo * max(n.value, 1))()
+        _lib.check(_lib.lib().clifft_svm_batch_info(self.handle, n.value, arr, ctypes.byref(n)), "batch_info")
+        kinds = ["wave", "lds", "global", "lane"]
+        out = []
+        for g in arr[:n.value]:
+            d = {f: getattr(g, f) for f, _ in g._fields_}
+            d["kind"] = kinds[d["kind"]]
+            out.append(d)
+        return out
+
+    def time_groups(self, enable: bool = True):
+        """Bracket each group's launch of later eager runs with timing events."""
+        _lib.check(_lib.lib().clifft_svm_batch_time_groups(self.handle, int(bool(enable))), "batch_time_groups")
+
+    def group_times(self) -> list:
+        """Kernel ms of each group in the last timed eager run."""
+        n = ctypes.c_uint32()
+        g = len(self.groups())
+        ms = (ctypes.c_float * max(g, 1))()
+        _lib.check(_lib.lib().clifft_svm_batch_group_times(self.handle, g, ms, ctypes.byref(n)), "batch_group_times")
+        return [float(x) for x in ms[:n.value]]
 
     @property
     def tuning(self) -> dict:

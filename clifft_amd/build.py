@@ -3,7 +3,12 @@
     python -m clifft_amd.build   # clifft_amd/libclifft_svm.so + libclifft_compile.so
 
 The library is compiled with hipcc for gfx950 with packed-FP32 instruction
-selection disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+selection disabled for the device code (-Xclang -target-feature -Xclang
+-packed-fp32-ops).  hipcc hands the flag to both compilation passes; the
+x86 host pass does not know the feature and prints "'-packed-fp32-ops' is
+not a recognized feature for this target (ignoring feature)", while the
+gfx950 device pass applies it: a device-only build of float2 fma code gives
+v_pk_fma_f32 without the flag and two v_fma_f32 with it.
 """
 
 from __future__ import annotations

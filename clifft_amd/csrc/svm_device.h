@@ -214,25 +214,24 @@ __host__ __device__ inline uint32_t lane_wave_bytes(uint32_t peak_rank, uint32_t
 }
 
 // Batched executor: one work-queue entry is a contiguous range of shots of
-// one program, run by one workgroup with the team shape named by `tier`.
-enum : uint32_t { TIER_WAVE = 0, TIER_LDS = 1, TIER_GLOBAL = 2, TIER_LANE = 3 };
-
+// one program, run by one team of its launch group.
 struct BatchItem {
     uint32_t prog;  // index into BatchArgs::progs / args / seeds
-    uint32_t tier;
     uint32_t shot_lo, shot_hi;
+    uint32_t pad;
 };
 
+// Arguments of one launch group of a batch.
 struct BatchArgs {
-    const BatchItem* items;          // sorted by decreasing estimated cost
+    const BatchItem* items;          // this group's items, by decreasing estimated cost
     const DevProgram* const* progs;  // device copy of each program's DevProgram
-    const ShotArgs* args;            // per program (outputs, rec_stride, amp_elems)
+    const ShotArgs* args;            // per program (outputs; the group's amp_elems and rec_stride)
     const uint64_t* seeds;           // per program
-    uint32_t* counter;               // queue head, cleared by a fill kernel before each launch
-    dv2* gscratch;                   // TIER_GLOBAL amplitudes, gscratch_elems per workgroup
-    uint64_t gscratch_elems;
+    uint32_t* counter;               // this group's queue head, cleared by a fill kernel before each launch
+    dv2* gscratch;                   // global-amplitude groups: team_elems per team
+    uint64_t team_elems;             // amp_elems x amp_slices of the group
     uint32_t n_items;
-    uint32_t pad;
+    uint32_t wave_stride;            // lane groups: LDS bytes per wavefront
 };
 
 // Read-only program tables are dereferenced through constant-address-space

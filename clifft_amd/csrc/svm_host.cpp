@@ -90,6 +90,11 @@ int validate(const clifft_program_view* v, std::vector<int32_t>& khist, bool& ha
     if (!v) return fail(CLIFFT_E_INVALID, "null program view");
     if (v->abi_version != CLIFFT_SVM_ABI_VERSION) return fail(CLIFFT_E_INVALID, "ABI version mismatch");
     if (v->peak_rank >= 63) return fail(CLIFFT_E_INVALID, "peak_rank >= 63 would cause undefined behavior");
+    // Amplitude indices are 32-bit in every sweep (2^k and 2 * 2^(k-1) stay
+    // below 2^31); a 2^30-amplitude state is 16 GiB.
+    if (v->peak_rank > kMaxPeakRank)
+        return fail(CLIFFT_E_UNSUPPORTED, "device executor supports peak_rank <= " + std::to_string(kMaxPeakRank) +
+                                              " (32-bit amplitude indices)");
     const uint32_t nq = v->num_qubits > 0 ? v->num_qubits : std::max<uint32_t>(v->peak_rank, 1u);
     const uint32_t words = (nq + 63) / 64;
     if (v->words != words) return fail(CLIFFT_E_INVALID, "words != ceil(num_qubits / 64)");
@@ -533,13 +538,21 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
         L.block = global_team_threads() ? global_team_threads() : 1024;
         L.global_amps = true;
     }
-    const uint32_t waves = L.block / 64;
     L.rec_stride = (uint32_t)rec_stride;
-    size_t lds = red + waves * rec_stride;
+    size_t lds = red + (L.block / 64) * rec_stride;
     if (!L.global_amps) lds += L.teams_per_block() * amp_bytes;
-    if (lds > 160 * 1024) {  // records too large next to LDS amplitudes
+    if (lds > kLdsBytes) {  // records too large next to LDS amplitudes
         L.global_amps = true;
-        lds = red + waves * rec_stride;
+        L.block_team = true;
+        L.block = global_team_threads() ? global_team_threads() : 1024;
+        lds = red + (L.block / 64) * rec_stride;
+    }
+    // every wavefront keeps its own record copy: fewer wavefronts per team
+    // when the copies do not fit (a shape with lds_bytes > kLdsBytes is
+    // refused at program creation)
+    while (L.global_amps && lds > kLdsBytes && L.block > 256) {
+        L.block /= 2;
+        lds = red + (L.block / 64) * rec_stride;
     }
     // Workgroup teams on global amplitudes get a second slice and an LDS
     // list for fused permutation/phase runs (one read and one write of the
@@ -547,7 +560,7 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
     // Only programs with a run of two or more, and states of at most 2^24
     // amplitudes (a second 512 MiB slice per team at most).
     if (L.global_amps && L.block_team && fuse_mono_enabled() && p->has_mono_runs && k <= 24 &&
-        lds + kMonoBytes <= 160 * 1024) {
+        lds + kMonoBytes <= kLdsBytes) {
         L.amp_slices = 2;
         lds += kMonoBytes;
     }
@@ -601,6 +614,7 @@ int choose_global_team(LaunchShape& L, uint64_t shots) {
         LaunchShape c = L;
         c.block = t;
         c.lds_bytes = kTeamScratchBytes + (t / 64) * (size_t)L.rec_stride + (L.amp_slices == 2 ? kMonoBytes : 0);
+        if (c.lds_bytes > kLdsBytes) break;  // larger teams hold more record copies
         int per_cu = 0;
         hipError_t e = shot_occupancy(c, &per_cu);
         if (e != hipSuccess) return fail(CLIFFT_E_HIP, std::string("occupancy query: ") + hipGetErrorString(e));
@@ -911,6 +925,14 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         p->dp_forced_dev = p->dp_dev + 1;
     }
     p->base_shape = plan_shape(p);
+    if (p->base_shape.lds_bytes > kLdsBytes) {
+        const std::string msg = "records of " + std::to_string(p->total_slots + p->num_det + p->num_obs) +
+                                " bytes per shot do not fit the LDS of a 256-thread team (one copy per "
+                                "wavefront; at most " +
+                                std::to_string((kLdsBytes - kTeamScratchBytes - kMonoBytes) / 4) + " bytes)";
+        clifft_svm_program_destroy(p);
+        return fail(CLIFFT_E_UNSUPPORTED, msg);
+    }
     *out = p;
     return CLIFFT_OK;
 }
@@ -954,6 +976,8 @@ static int run_shots(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint3
         const uint64_t n = std::min(chunk_max, shots - base);
         const bool kfault = mode == MODE_KFAULT;
         LaunchShape L = kfault ? plan_shape(p, false, true) : p->base_shape;
+        if (L.lds_bytes > kLdsBytes)
+            return fail(CLIFFT_E_UNSUPPORTED, "records and k-fault scratch do not fit the LDS of a team");
         void* gs = nullptr;
         int rc = finalize_shape(p, L, n, st, &gs);
         if (rc) return rc;
@@ -1259,6 +1283,7 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
     if (s->exp_vals && p->num_ev) HIP_TRY(hipMemcpyAsync(d_ev, s->exp_vals, p->num_ev * 8, hipMemcpyHostToDevice, st));
     if (s->forced_record) HIP_TRY(hipMemcpyAsync(d_forced, s->forced_record, p->total_slots, hipMemcpyHostToDevice, st));
     LaunchShape L = p->base_shape.lane ? plan_shape(p, false) : p->base_shape;
+    if (L.lds_bytes > kLdsBytes) return fail(CLIFFT_E_UNSUPPORTED, "records do not fit the LDS of a team");
     void* gs = nullptr;
     rc = finalize_shape(p, L, 1, st, &gs);
     if (rc) return rc;
@@ -1302,41 +1327,61 @@ int clifft_svm_execute(clifft_svm_program* p, clifft_svm_state* s) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Batched persistent executor
+// Batched execution
 // ---------------------------------------------------------------------------
-// The persistent multi-program launch is switched off: its first device run
-// ended in a GPU memory fault whose cause has not been found (a CPU
-// emulation of the same programs under AddressSanitizer found no
-// out-of-bounds access).  Until a cause is shown, every entry point that
-// would launch it returns CLIFFT_E_UNSUPPORTED; clifft_svm_batch_sample with
-// CLIFFT_BATCH_PER_PROGRAM runs the same programs with one plain launch each.
-constexpr bool kPersistentBatchEnabled = false;
-constexpr const char* kPersistentBatchOff =
-    "the persistent batch launch is switched off in this build (an unexplained GPU memory fault); "
-    "use CLIFFT_BATCH_PER_PROGRAM";
+// A batch is cut into launch groups, one per kernel shape: the lane kernel
+// (by frame-word width), wavefront teams, LDS workgroup teams (by team size)
+// and global-amplitude workgroup teams.  Each group is one persistent launch
+// whose teams drain the group's device queue of work items; the groups run
+// one after another on the caller's stream, or side by side on the batch's
+// own streams (tuning.concurrent).  Every program belongs to one group, and
+// its ShotArgs carry the group's largest amplitude count and record stride,
+// so one LDS carve-up serves every item of the launch.  batch_audit checks
+// every item against its launch's allocations on the host before anything
+// is uploaded.
+
+struct BatchGroup {
+    LaunchShape L;               // kernel shape; grid, block, lds_bytes of the launch
+    std::vector<uint32_t> progs; // member program indices
+    uint32_t amp_elems = 1;      // team tiers: largest 2^peak_rank of the members
+    uint32_t rec_stride = 0;     // team tiers: largest record stride
+    uint32_t wave_stride = 0;    // lane tier: largest lane_wave_bytes
+    uint32_t amp_slices = 1;     // global tier: 2 when a member runs fused permutation/phase runs
+    uint32_t teams_resident = 0; // teams in flight (grid x teams per workgroup)
+    std::vector<BatchItem> items;
+    size_t items_off = 0;        // offset of this group's items in the plan allocation
+    void* gscratch = nullptr;
+    size_t gscratch_bytes = 0;
+};
 
 struct clifft_svm_batch {
     std::vector<clifft_svm_program*> progs;
     std::vector<uint64_t> shots;
-    std::vector<ShotArgs> args;  // host copy of the per-program launch arguments
+    std::vector<clifft_svm_records> outs;
     uint32_t flags = 0;
-    int nw = 1;
     clifft_svm_batch_tuning tuning{};
-    bool tuned_valid = false;
-    // device layout: [BatchArgs | counter | progs | ShotArgs[n] | seeds[n]], items separately
+    std::vector<BatchGroup> groups;
+    std::vector<uint32_t> group_of;  // per program
+    std::vector<ShotArgs> args;      // per program, for the current plan
+    // device: [program pointers | ShotArgs[n] | seeds[n]]
     void* dev = nullptr;
-    BatchItem* items_dev = nullptr;
-    size_t items_cap = 0;
-    uint32_t n_items = 0;
-    BatchArgs* args_dev = nullptr;
-    uint32_t* counter_dev = nullptr;
+    const DevProgram** progs_dev = nullptr;
+    ShotArgs* args_dev = nullptr;
     uint64_t* seeds_dev = nullptr;
-    void* gscratch = nullptr;
-    size_t gscratch_bytes = 0;
-    uint32_t grid = 0;
-    uint32_t block = 256;
-    size_t lds_bytes = 0;
-    uint64_t seed_cache_valid = 0;
+    // device, per plan: [counters[G] | BatchArgs[G] | items of every group]
+    void* plan_dev = nullptr;
+    size_t plan_dev_bytes = 0;
+    uint32_t* counters_dev = nullptr;
+    BatchArgs* bargs_dev = nullptr;
+    // side streams of concurrent groups and their join events
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_done;
+    hipEvent_t fork = nullptr;
+    // optional per-group kernel timing of eager runs (timing events on the
+    // stream each group's launch goes to)
+    bool time_groups = false;
+    std::vector<hipEvent_t> t0, t1;
+    bool timed_valid = false;
     // captured graph
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -1352,133 +1397,406 @@ struct clifft_svm_batch {
 
 namespace {
 
-uint32_t tier_of(const clifft_svm_program* p, const clifft_svm_batch_tuning& t) {
-    if (p->words == 1 && (int32_t)p->peak_rank <= t.lane_rank_max) return TIER_LANE;
-    if (p->peak_rank <= t.wave_rank_max) return TIER_WAVE;
-    if (p->peak_rank <= t.lds_rank_max) return TIER_LDS;
-    return TIER_GLOBAL;
-}
-
-size_t lds_need(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
-    const size_t waves = block / 64;
-    if (tier == TIER_LANE) return waves * lane_wave_bytes(p->peak_rank, p->total_slots, p->num_det, p->num_obs);
-    size_t lds = kTeamScratchBytes + waves * rec_stride_of(p);
-    if (tier == TIER_WAVE) lds += waves * ((size_t)16 << p->peak_rank);
-    if (tier == TIER_LDS) lds += (size_t)16 << p->peak_rank;
-    return lds;
-}
-
-// Estimated workgroup time of one shot (arbitrary units): the interpreter's
-// per-instruction issue cost plus the state traffic spread over the team.
-double shot_cost(const clifft_svm_program* p, uint32_t tier, uint32_t block) {
-    if (tier == TIER_LANE)  // 64 shots per wavefront, the workgroup's waves share an item
-        return ((double)p->host_bc.size() + p->bytes_per_shot / 64.0) / (double)block;
-    const double lanes = tier == TIER_WAVE ? 64.0 : (double)block;
-    const double per_byte = tier == TIER_GLOBAL ? 1.0 / 8.0 : 1.0 / 32.0;
-    double c = (double)p->host_bc.size() + p->bytes_per_shot / lanes * per_byte;
-    if (tier == TIER_WAVE) c /= (double)(block / 64);  // the workgroup's waves share an item
-    return c;
+clifft_svm_batch_tuning default_tuning() {
+    clifft_svm_batch_tuning t{};
+    t.global_threads = 256;
+    t.wave_rank_max = 7;
+    t.lds_rank_max = (uint32_t)lds_max_rank();
+    t.wave_chunk = 16;
+    t.block_chunk = 4;
+    t.concurrent = 1;
+    t.lane_rank_max = lane_max_rank() < 0 ? -1 : std::min(lane_max_rank(), 6);
+    t.lane_chunk = 256;
+    return t;
 }
 
 int batch_check_tuning(const clifft_svm_batch_tuning& t) {
-    if (t.block_threads != 256 && t.block_threads != 512)
-        return fail(CLIFFT_E_INVALID, "batch tuning: block_threads must be 256 or 512");
+    if (t.global_threads != 256 && t.global_threads != 512 && t.global_threads != 1024)
+        return fail(CLIFFT_E_INVALID, "batch tuning: global_threads must be 256, 512 or 1024");
     if (t.wave_rank_max > 8 || t.lds_rank_max > 13 || t.lds_rank_max < t.wave_rank_max)
         return fail(CLIFFT_E_INVALID, "batch tuning: need wave_rank_max <= lds_rank_max, wave_rank_max <= 8, lds_rank_max <= 13");
     if (t.wave_chunk == 0 || t.block_chunk == 0 || t.lane_chunk == 0)
         return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
     if (t.lane_rank_max > 6) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 6");
+    if (t.concurrent > 1) return fail(CLIFFT_E_INVALID, "batch tuning: concurrent must be 0 or 1");
     return CLIFFT_OK;
 }
 
-// Build the work queue and launch geometry for the batch's tuning, capping
-// each program at `cap` shots (the autotuner's sample); upload it.
-int batch_plan(clifft_svm_batch* b, uint64_t cap, hipStream_t st) {
+// Kernel shape of one program in a batch (the plain planner's choice with
+// the tuning's rank thresholds).  Returns false if its records do not fit.
+bool batch_shape(const clifft_svm_program* p, const clifft_svm_batch_tuning& t, LaunchShape& L) {
+    L = LaunchShape();
+    L.nw = (int)p->nw;
+    const uint32_t k = p->peak_rank;
+    const size_t red = kTeamScratchBytes, rec = rec_stride_of(p);
+    if (p->words == 1 && (int32_t)k <= t.lane_rank_max &&
+        lane_wave_bytes(k, p->total_slots, p->num_det, p->num_obs) <= kBatchMaxDynLds) {
+        L.lane = true;
+        L.lane_narrow = p->frame32;
+        return true;
+    }
+    if (k <= t.wave_rank_max && red + 4 * (rec + ((size_t)16 << k)) <= kBatchMaxDynLds) {
+        L.block = 256;
+        return true;
+    }
+    if (k <= t.lds_rank_max) {
+        const uint32_t waves = k <= 9 ? 2 : (k <= 10 ? 4 : (k <= 12 ? 8 : 16));
+        if (red + waves * rec + ((size_t)16 << k) <= kBatchMaxDynLds) {
+            L.block_team = true;
+            L.block = 64 * waves;
+            return true;
+        }
+    }
+    L.block_team = true;
+    L.global_amps = true;
+    L.block = t.global_threads;
+    while (red + (L.block / 64) * rec > kBatchMaxDynLds && L.block > 256) L.block /= 2;
+    return red + (L.block / 64) * rec <= kBatchMaxDynLds;
+}
+
+bool same_kernel(const LaunchShape& a, const LaunchShape& b) {
+    return a.lane == b.lane && a.lane_narrow == b.lane_narrow && a.nw == b.nw && a.block_team == b.block_team &&
+           a.global_amps == b.global_amps && (a.lane || a.block == b.block);
+}
+
+bool mono_eligible(const clifft_svm_program* p) {
+    return fuse_mono_enabled() && p->has_mono_runs && p->peak_rank <= 24;
+}
+
+// Dynamic LDS of a group's launch from its largest members.
+size_t group_lds(const BatchGroup& g) {
+    const LaunchShape& L = g.L;
+    if (L.lane) return (size_t)(L.block / 64) * g.wave_stride;
+    size_t lds = kTeamScratchBytes + (size_t)(L.block / 64) * g.rec_stride;
+    if (!L.global_amps) lds += (size_t)L.teams_per_block() * g.amp_elems * 16;
+    if (L.global_amps && g.amp_slices == 2) lds += kMonoBytes;
+    return lds;
+}
+
+void group_sizes(BatchGroup& g, const std::vector<clifft_svm_program*>& progs) {
+    g.amp_elems = 1;
+    g.rec_stride = 0;
+    g.wave_stride = 0;
+    g.amp_slices = 1;
+    for (uint32_t i : g.progs) {
+        const clifft_svm_program* p = progs[i];
+        g.amp_elems = std::max(g.amp_elems, 1u << p->peak_rank);
+        g.rec_stride = std::max(g.rec_stride, (uint32_t)rec_stride_of(p));
+        g.wave_stride = std::max(g.wave_stride, lane_wave_bytes(p->peak_rank, p->total_slots, p->num_det, p->num_obs));
+    }
+    if (g.L.lane) {
+        const uint32_t waves = (uint32_t)std::min<size_t>(4, kBatchMaxDynLds / std::max<uint32_t>(g.wave_stride, 1));
+        g.L.block = 64 * std::max<uint32_t>(waves, 1);
+        g.L.lds_bytes = group_lds(g);
+        g.L.lane_dense = 6 * g.L.lds_bytes <= kLdsBytes;
+        return;
+    }
+    if (g.L.global_amps) {
+        bool any = false;
+        for (uint32_t i : g.progs) any |= mono_eligible(progs[i]);
+        g.amp_slices = any ? 2 : 1;
+        if (any && group_lds(g) > kBatchMaxDynLds) g.amp_slices = 1;
+    }
+    g.L.amp_slices = g.amp_slices;
+    g.L.rec_stride = g.rec_stride;
+    g.L.lds_bytes = group_lds(g);
+}
+
+// Estimated team time of one shot (arbitrary units): the interpreter's
+// per-instruction issue cost plus the state traffic spread over the team.
+double shot_cost(const clifft_svm_program* p, const LaunchShape& L) {
+    if (L.lane) return ((double)p->host_bc.size() + p->bytes_per_shot / 64.0) / 64.0;
+    const double lanes = L.block_team ? (double)L.block : 64.0;
+    const double per_byte = L.global_amps ? 1.0 / 8.0 : 1.0 / 32.0;
+    return (double)p->host_bc.size() + p->bytes_per_shot / lanes * per_byte;
+}
+
+// Group the programs and cut their shots (at most `cap` per program) into
+// items, each group's sorted by decreasing estimated cost.
+int batch_group(clifft_svm_batch* b, uint64_t cap) {
     const clifft_svm_batch_tuning& t = b->tuning;
-    int rc = batch_check_tuning(t);
-    if (rc) return rc;
+    std::vector<BatchGroup> groups;
+    std::vector<uint32_t> group_of(b->progs.size());
+    std::vector<LaunchShape> shape(b->progs.size());
+    for (size_t i = 0; i < b->progs.size(); ++i) {
+        if (!batch_shape(b->progs[i], t, shape[i]))
+            return fail(CLIFFT_E_UNSUPPORTED, "batch: the records of program " + std::to_string(i) +
+                                                  " do not fit the LDS of a 256-thread team");
+        size_t gi = 0;
+        while (gi < groups.size() && !same_kernel(groups[gi].L, shape[i])) ++gi;
+        if (gi == groups.size()) {
+            groups.emplace_back();
+            groups.back().L = shape[i];
+        }
+        groups[gi].progs.push_back((uint32_t)i);
+    }
+    // A team group whose largest amplitudes and largest records do not fit
+    // together hands its largest-record members to the global group.
+    std::vector<uint32_t> evicted;
+    for (BatchGroup& g : groups) {
+        if (g.L.lane || g.L.global_amps) continue;
+        std::stable_sort(g.progs.begin(), g.progs.end(), [&](uint32_t x, uint32_t y) {
+            return rec_stride_of(b->progs[x]) > rec_stride_of(b->progs[y]);
+        });
+        group_sizes(g, b->progs);
+        while (g.L.lds_bytes > kBatchMaxDynLds && g.progs.size() > 1) {
+            evicted.push_back(g.progs.front());
+            g.progs.erase(g.progs.begin());
+            group_sizes(g, b->progs);
+        }
+    }
+    for (uint32_t i : evicted) {
+        LaunchShape L;
+        L.nw = (int)b->progs[i]->nw;
+        L.block_team = true;
+        L.global_amps = true;
+        L.block = t.global_threads;
+        const size_t rec = rec_stride_of(b->progs[i]);
+        while (kTeamScratchBytes + (L.block / 64) * rec > kBatchMaxDynLds && L.block > 256) L.block /= 2;
+        size_t gi = 0;
+        while (gi < groups.size() && !same_kernel(groups[gi].L, L)) ++gi;
+        if (gi == groups.size()) {
+            groups.emplace_back();
+            groups.back().L = L;
+        }
+        groups[gi].progs.push_back(i);
+    }
+    std::vector<BatchGroup> kept;
+    for (BatchGroup& g : groups) {
+        if (g.progs.empty()) continue;
+        group_sizes(g, b->progs);
+        if (g.L.lds_bytes > kBatchMaxDynLds)
+            return fail(CLIFFT_E_UNSUPPORTED, "batch: a launch group's records do not fit the LDS");
+        kept.push_back(std::move(g));
+    }
+    groups.swap(kept);
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        BatchGroup& g = groups[gi];
+        struct Pending {
+            BatchItem it;
+            double cost;
+        };
+        std::vector<Pending> work;
+        const uint32_t chunk = g.L.lane ? std::max<uint32_t>(64, t.lane_chunk / 64 * 64)
+                                        : (g.L.block_team ? t.block_chunk : t.wave_chunk);
+        for (uint32_t i : g.progs) {
+            group_of[i] = (uint32_t)gi;
+            const uint64_t n = std::min<uint64_t>(b->shots[i], cap);
+            const double c = shot_cost(b->progs[i], g.L);
+            for (uint64_t lo = 0; lo < n; lo += chunk) {
+                const uint64_t hi = std::min<uint64_t>(n, lo + chunk);
+                Pending w;
+                w.it.prog = i;
+                w.it.shot_lo = (uint32_t)lo;
+                w.it.shot_hi = (uint32_t)hi;
+                w.it.pad = 0;
+                w.cost = c * (double)(hi - lo);
+                work.push_back(w);
+            }
+        }
+        std::stable_sort(work.begin(), work.end(), [](const Pending& x, const Pending& y) { return x.cost > y.cost; });
+        g.items.resize(work.size());
+        for (size_t j = 0; j < work.size(); ++j) g.items[j] = work[j].it;
+    }
+    b->groups.swap(groups);
+    b->group_of.swap(group_of);
+    return CLIFFT_OK;
+}
+
+// Host-side bounds audit of a plan: every item's program, shot range, LDS
+// carve-up and global slice against the allocations its launch gets.
+int batch_audit(const clifft_svm_batch* b) {
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        const BatchGroup& g = b->groups[gi];
+        const LaunchShape& L = g.L;
+        const std::string where = "batch plan, group " + std::to_string(gi) + ": ";
+        if (L.lds_bytes > kBatchMaxDynLds) return fail(CLIFFT_E_INVALID, where + "LDS over the limit");
+        if (L.block == 0 || L.block > 1024 || L.block % 64) return fail(CLIFFT_E_INVALID, where + "bad block");
+        if (!L.block_team && !L.lane && L.block > 256) return fail(CLIFFT_E_INVALID, where + "bad wave-team block");
+        if (L.global_amps && (size_t)g.teams_resident * g.amp_elems * g.amp_slices * 16 > g.gscratch_bytes)
+            return fail(CLIFFT_E_INVALID, where + "global slices exceed the scratch");
+        for (const BatchItem& it : g.items) {
+            if (it.prog >= b->progs.size() || b->group_of[it.prog] != gi)
+                return fail(CLIFFT_E_INVALID, where + "item names a program outside the group");
+            const clifft_svm_program* p = b->progs[it.prog];
+            const ShotArgs& A = b->args[it.prog];
+            if (it.shot_lo >= it.shot_hi || it.shot_hi > b->shots[it.prog])
+                return fail(CLIFFT_E_INVALID, where + "item shot range outside the program's shots");
+            if (L.lane) {
+                const size_t need = lane_wave_bytes(p->peak_rank, p->total_slots, p->num_det, p->num_obs);
+                if (need > g.wave_stride || (size_t)(L.block / 64) * g.wave_stride > L.lds_bytes)
+                    return fail(CLIFFT_E_INVALID, where + "lane columns exceed the wave's LDS slice");
+                continue;
+            }
+            if (A.amp_elems != g.amp_elems || A.rec_stride != g.rec_stride)
+                return fail(CLIFFT_E_INVALID, where + "ShotArgs carve-up differs from the group's");
+            if ((1u << p->peak_rank) > A.amp_elems || rec_stride_of(p) > A.rec_stride)
+                return fail(CLIFFT_E_INVALID, where + "program exceeds the group's amplitude or record room");
+            if (p->total_slots + p->num_det + p->num_obs > A.rec_stride)
+                return fail(CLIFFT_E_INVALID, where + "records exceed the record stride");
+            size_t carve = kTeamScratchBytes + (size_t)(L.block / 64) * A.rec_stride;
+            if (!L.global_amps) carve += (size_t)L.teams_per_block() * A.amp_elems * 16;
+            if (L.global_amps && A.amp_slices == 2) carve += kMonoBytes;
+            if (carve > L.lds_bytes) return fail(CLIFFT_E_INVALID, where + "LDS carve-up exceeds the launch's LDS");
+            if (L.global_amps && A.amp_slices > g.amp_slices)
+                return fail(CLIFFT_E_INVALID, where + "program uses more slices than a team holds");
+            if (!L.global_amps && A.amp_slices != 1)
+                return fail(CLIFFT_E_INVALID, where + "LDS tier with a second slice");
+        }
+    }
+    return CLIFFT_OK;
+}
+
+// Teams of a launch group that stay resident at once, from the kernel's
+// occupancy; the global group is also held to the device memory budget.
+int batch_grids(clifft_svm_batch* b) {
     DeviceInfo& di = device_info();
-    const uint32_t block = t.block_threads;
-    size_t lds = kTeamScratchBytes;
-    uint32_t kglobal = 0;
-    bool any_global = false;
-    struct Pending {
-        BatchItem it;
-        double cost;
-    };
-    std::vector<Pending> work;
+    for (BatchGroup& g : b->groups) {
+        int per_cu = 0;
+        HIP_TRY(batch_occupancy(g.L, &per_cu));
+        if (per_cu <= 0) return fail(CLIFFT_E_UNSUPPORTED, "batch: a launch group does not fit a compute unit");
+        uint64_t grid = (uint64_t)per_cu * (uint64_t)di.num_cus;
+        const uint64_t tpb = g.L.lane ? g.L.block / 64 : g.L.teams_per_block();  // queue consumers per workgroup
+        grid = std::min<uint64_t>(grid, std::max<uint64_t>(1, (g.items.size() + tpb - 1) / tpb));
+        if (g.L.global_amps) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 8ull << 30;
+            size_t budget = free_b + g.gscratch_bytes;
+            const size_t headroom = std::max<size_t>(budget * 2 / 5, 16ull << 30);
+            budget = budget > headroom ? budget - headroom : (size_t)1 << 30;
+            budget = std::min<size_t>(budget, 192ull << 30);
+            const size_t per_team = (size_t)g.amp_elems * g.amp_slices * 16;
+            grid = std::min<uint64_t>(grid, std::max<uint64_t>(1, budget / per_team));
+            const size_t need = (size_t)grid * per_team;
+            if (need > g.gscratch_bytes) {
+                if (g.gscratch) {
+                    HIP_TRY(hipDeviceSynchronize());
+                    (void)hipFree(g.gscratch);
+                }
+                g.gscratch = nullptr;
+                g.gscratch_bytes = 0;
+                HIP_TRY(hipMalloc(&g.gscratch, need));
+                g.gscratch_bytes = need;
+            }
+        }
+        g.L.grid = (uint32_t)grid;
+        g.teams_resident = (uint32_t)(grid * (g.L.lane ? 1 : g.L.teams_per_block()));
+    }
+    return CLIFFT_OK;
+}
+
+void free_group_scratch(std::vector<BatchGroup>& groups) {
+    for (BatchGroup& g : groups)
+        if (g.gscratch) (void)hipFree(g.gscratch);
+}
+
+// Plan the batch for its tuning (each program capped at `cap` shots, the
+// autotuner's sample), audit the plan and upload it.
+int batch_plan(clifft_svm_batch* b, uint64_t cap, hipStream_t st) {
+    int rc = batch_check_tuning(b->tuning);
+    if (rc) return rc;
+    // launches of the previous plan may still read its items and arguments
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<BatchGroup> old;
+    old.swap(b->groups);
+    rc = batch_group(b, cap);
+    if (rc) {
+        b->groups.swap(old);
+        return rc;
+    }
+    // keep the global scratch of the previous plan when its shape matches
+    for (BatchGroup& g : b->groups)
+        for (BatchGroup& o : old)
+            if (o.gscratch && same_kernel(o.L, g.L) && g.L.global_amps) {
+                g.gscratch = o.gscratch;
+                g.gscratch_bytes = o.gscratch_bytes;
+                o.gscratch = nullptr;
+            }
+    free_group_scratch(old);
+    rc = batch_grids(b);
+    if (rc) return rc;
+    // per-program launch arguments with the group's carve-up
     for (size_t i = 0; i < b->progs.size(); ++i) {
         clifft_svm_program* p = b->progs[i];
-        uint32_t tier = tier_of(p, t);
-        if (tier == TIER_LANE && lds_need(p, tier, block) > kBatchMaxDynLds)  // lanes do not fit: next tier
-            tier = p->peak_rank <= t.wave_rank_max ? TIER_WAVE : (p->peak_rank <= t.lds_rank_max ? TIER_LDS : TIER_GLOBAL);
-        if (tier != TIER_GLOBAL && lds_need(p, tier, block) > kBatchMaxDynLds) tier = TIER_GLOBAL;
-        lds = std::max(lds, lds_need(p, tier, block));
-        if (tier == TIER_GLOBAL) {
-            any_global = true;
-            kglobal = std::max(kglobal, p->peak_rank);
-        }
-        const uint64_t n = std::min<uint64_t>(b->shots[i], cap);
-        const uint32_t chunk = tier == TIER_LANE ? t.lane_chunk : (tier == TIER_WAVE ? t.wave_chunk : t.block_chunk);
-        const double c = shot_cost(p, tier, block);
-        for (uint64_t lo = 0; lo < n; lo += chunk) {
-            uint64_t hi = std::min<uint64_t>(n, lo + chunk);
-            Pending w;
-            w.it.prog = (uint32_t)i;
-            w.it.tier = tier;
-            w.it.shot_lo = (uint32_t)lo;
-            w.it.shot_hi = (uint32_t)hi;
-            w.cost = c * (double)(hi - lo);
-            work.push_back(w);
+        const BatchGroup& g = b->groups[b->group_of[i]];
+        ShotArgs& A = b->args[i];
+        fill_common_args(p, A, nullptr);
+        A.shot_begin = 0;
+        A.shot_count = b->shots[i];
+        A.mode = MODE_SAMPLE;
+        A.flags = b->flags;
+        A.out_meas = b->outs[i].measurements;
+        A.out_det = b->outs[i].detectors;
+        A.out_obs = b->outs[i].observables;
+        A.out_ev = b->outs[i].exp_vals;
+        A.out_disc = b->outs[i].discarded;
+        if (!g.L.lane) {
+            A.amp_elems = g.amp_elems;
+            A.rec_stride = g.rec_stride;
+            A.amp_slices = g.L.global_amps && g.amp_slices == 2 && mono_eligible(p) ? 2u : 1u;
         }
     }
-    std::stable_sort(work.begin(), work.end(), [](const Pending& a, const Pending& c) { return a.cost > c.cost; });
-    int per_cu = 0;
-    HIP_TRY(batch_occupancy(b->nw, block, lds, &per_cu));
-    if (per_cu <= 0) return fail(CLIFFT_E_UNSUPPORTED, "batch: launch shape does not fit a compute unit");
-    uint32_t grid = t.grid ? t.grid : (uint32_t)(per_cu * di.num_cus);
-    grid = std::max<uint32_t>(1, std::min<uint64_t>(grid, std::max<size_t>(work.size(), 1)));
-    const size_t gelems = any_global ? ((size_t)1 << kglobal) : 0;
-    const size_t gbytes = (size_t)grid * gelems * 16;
-    if (gbytes > b->gscratch_bytes) {
-        if (b->gscratch) {
+    rc = batch_audit(b);
+    if (rc) return rc;
+    const size_t G = b->groups.size();
+    const size_t o_bargs = align_up(G * 4, 256);
+    size_t o_items = o_bargs + align_up(G * sizeof(BatchArgs), 256);
+    size_t total = o_items;
+    for (BatchGroup& g : b->groups) {
+        g.items_off = total;
+        total += align_up(std::max<size_t>(g.items.size(), 1) * sizeof(BatchItem), 256);
+    }
+    if (total > b->plan_dev_bytes) {
+        if (b->plan_dev) {
             HIP_TRY(hipDeviceSynchronize());
-            (void)hipFree(b->gscratch);
+            (void)hipFree(b->plan_dev);
         }
-        b->gscratch = nullptr;
-        b->gscratch_bytes = 0;
-        HIP_TRY(hipMalloc(&b->gscratch, gbytes));
-        b->gscratch_bytes = gbytes;
+        b->plan_dev = nullptr;
+        b->plan_dev_bytes = 0;
+        HIP_TRY(hipMalloc(&b->plan_dev, total));
+        b->plan_dev_bytes = total;
     }
-    if (work.size() > b->items_cap) {
-        if (b->items_dev) {
-            HIP_TRY(hipDeviceSynchronize());
-            (void)hipFree(b->items_dev);
-        }
-        b->items_dev = nullptr;
-        b->items_cap = 0;
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&b->items_dev), std::max<size_t>(work.size(), 1) * sizeof(BatchItem)));
-        b->items_cap = std::max<size_t>(work.size(), 1);
+    auto* base = static_cast<uint8_t*>(b->plan_dev);
+    b->counters_dev = reinterpret_cast<uint32_t*>(base);
+    b->bargs_dev = reinterpret_cast<BatchArgs*>(base + o_bargs);
+    std::vector<uint8_t> staging(total, 0);
+    for (size_t gi = 0; gi < G; ++gi) {
+        const BatchGroup& g = b->groups[gi];
+        BatchArgs B{};
+        B.items = reinterpret_cast<const BatchItem*>(base + g.items_off);
+        B.progs = b->progs_dev;
+        B.args = b->args_dev;
+        B.seeds = b->seeds_dev;
+        B.counter = b->counters_dev + gi;
+        B.gscratch = reinterpret_cast<dv2*>(g.gscratch);
+        B.team_elems = g.L.global_amps ? (uint64_t)g.amp_elems * g.amp_slices : 0;
+        B.n_items = (uint32_t)g.items.size();
+        B.wave_stride = g.wave_stride;
+        std::memcpy(staging.data() + o_bargs + gi * sizeof(BatchArgs), &B, sizeof(B));
+        if (!g.items.empty())
+            std::memcpy(staging.data() + g.items_off, g.items.data(), g.items.size() * sizeof(BatchItem));
     }
-    std::vector<BatchItem> items(work.size());
-    for (size_t i = 0; i < work.size(); ++i) items[i] = work[i].it;
-    if (!items.empty())
-        HIP_TRY(hipMemcpyAsync(b->items_dev, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
-    BatchArgs B{};
-    B.items = b->items_dev;
-    B.progs = reinterpret_cast<const DevProgram* const*>(static_cast<uint8_t*>(b->dev) + 256);
-    B.args = reinterpret_cast<const ShotArgs*>(static_cast<uint8_t*>(b->dev) + 256 + align_up(b->progs.size() * 8, 256));
-    B.seeds = b->seeds_dev;
-    B.counter = b->counter_dev;
-    B.gscratch = reinterpret_cast<dv2*>(b->gscratch);
-    B.gscratch_elems = gelems;
-    B.n_items = (uint32_t)items.size();
-    HIP_TRY(hipMemcpyAsync(b->args_dev, &B, sizeof(B), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->plan_dev, staging.data(), total, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->args_dev, b->args.data(), b->args.size() * sizeof(ShotArgs), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));  // host staging above is freed on return
-    b->n_items = (uint32_t)items.size();
-    b->grid = grid;
-    b->block = block;
-    b->lds_bytes = lds;
+    // side streams for concurrent groups (default priority)
+    while (b->side.size() < G) {
+        hipStream_t s2;
+        hipEvent_t ev;
+        HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        b->side.push_back(s2);
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->side_done.push_back(ev);
+    }
+    if (!b->fork) HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    while (b->t0.size() < G) {
+        hipEvent_t a, c;
+        HIP_TRY(hipEventCreate(&a));
+        b->t0.push_back(a);
+        HIP_TRY(hipEventCreate(&c));
+        b->t1.push_back(c);
+    }
+    b->timed_valid = false;
     b->epoch++;
     return CLIFFT_OK;
 }
@@ -1496,17 +1814,33 @@ int batch_upload_seeds(clifft_svm_batch* b, const uint64_t* seeds, hipStream_t s
     return CLIFFT_OK;
 }
 
-clifft_svm_batch_tuning default_tuning() {
-    clifft_svm_batch_tuning t{};
-    t.block_threads = 256;
-    t.wave_rank_max = 7;
-    t.lds_rank_max = 11;
-    t.wave_chunk = 32;
-    t.block_chunk = 8;
-    t.grid = 0;
-    t.lane_rank_max = lane_max_rank() < 0 ? -1 : std::min(lane_max_rank(), 6);
-    t.lane_chunk = 1024;
-    return t;
+// Queue resets and every group's launch, ordered after prior work on `st`
+// and before later work on it.
+int batch_enqueue(clifft_svm_batch* b, hipStream_t st, bool capturing = false) {
+    const uint32_t G = (uint32_t)b->groups.size();
+    if (G == 0) return CLIFFT_OK;
+    b->timed_valid = b->time_groups && !capturing;
+    HIP_TRY(clear_counters(b->counters_dev, G, st));
+    const bool conc = b->tuning.concurrent && G > 1;
+    if (conc) HIP_TRY(hipEventRecord(b->fork, st));
+    for (uint32_t gi = 0; gi < G; ++gi) {
+        const BatchGroup& g = b->groups[gi];
+        if (g.items.empty()) continue;
+        hipStream_t s = st;
+        if (conc) {
+            s = b->side[gi];
+            HIP_TRY(hipStreamWaitEvent(s, b->fork, 0));
+        }
+        const bool timed = b->time_groups && !capturing;
+        if (timed) HIP_TRY(hipEventRecord(b->t0[gi], s));
+        HIP_TRY(launch_batch_group(g.L, b->bargs_dev + gi, s));
+        if (timed) HIP_TRY(hipEventRecord(b->t1[gi], s));
+        if (conc) {
+            HIP_TRY(hipEventRecord(b->side_done[gi], s));
+            HIP_TRY(hipStreamWaitEvent(st, b->side_done[gi], 0));
+        }
+    }
+    return CLIFFT_OK;
 }
 
 }  // namespace
@@ -1520,8 +1854,6 @@ int clifft_svm_batch_create(clifft_svm_program* const* programs, const uint64_t*
     if (!programs || !shots || !outs || num_programs == 0) return fail(CLIFFT_E_INVALID, "null argument or empty batch");
     auto* b = new clifft_svm_batch();
     b->flags = flags | CLIFFT_REC_DEVICE;
-    const bool packed = (flags & CLIFFT_REC_PACKED) != 0;
-    b->nw = 1;
     for (uint32_t i = 0; i < num_programs; ++i) {
         clifft_svm_program* p = programs[i];
         if (!p) {
@@ -1536,45 +1868,30 @@ int clifft_svm_batch_create(clifft_svm_program* const* programs, const uint64_t*
             delete b;
             return fail(CLIFFT_E_UNSUPPORTED, "batch: more than 2^32-1 shots for one program");
         }
-        if (p->nw != 1) b->nw = 0;
         b->progs.push_back(p);
         b->shots.push_back(shots[i]);
-        ShotArgs A;
-        fill_common_args(p, A, nullptr);
-        A.shot_begin = 0;
-        A.shot_count = shots[i];
-        A.mode = MODE_SAMPLE;
-        A.flags = b->flags;
-        (void)packed;
-        A.out_meas = outs[i].measurements;
-        A.out_det = outs[i].detectors;
-        A.out_obs = outs[i].observables;
-        A.out_ev = outs[i].exp_vals;
-        A.out_disc = outs[i].discarded;
-        b->args.push_back(A);
+        b->outs.push_back(outs[i]);
     }
     const size_t n = num_programs;
-    const size_t o_progs = 256, o_args = o_progs + align_up(n * 8, 256);
+    b->args.resize(n);
+    const size_t o_args = align_up(n * 8, 256);
     const size_t o_seeds = o_args + align_up(n * sizeof(ShotArgs), 256);
-    const size_t o_counter = o_seeds + align_up(n * 8, 256);
-    const size_t total = o_counter + 256;
+    const size_t total = o_seeds + align_up(n * 8, 256);
     hipError_t e = hipMalloc(&b->dev, total);
     if (e != hipSuccess) {
         delete b;
         return fail(CLIFFT_E_HIP, std::string("batch upload: ") + hipGetErrorString(e));
     }
     auto* base = static_cast<uint8_t*>(b->dev);
-    b->args_dev = reinterpret_cast<BatchArgs*>(base);
+    b->progs_dev = reinterpret_cast<const DevProgram**>(base);
+    b->args_dev = reinterpret_cast<ShotArgs*>(base + o_args);
     b->seeds_dev = reinterpret_cast<uint64_t*>(base + o_seeds);
-    b->counter_dev = reinterpret_cast<uint32_t*>(base + o_counter);
-    std::vector<uint8_t> staging(o_seeds, 0);
+    std::vector<uint8_t> staging(total, 0);
     for (size_t i = 0; i < n; ++i) {
         const DevProgram* d = b->progs[i]->dp_dev;
-        std::memcpy(staging.data() + o_progs + i * 8, &d, 8);
+        std::memcpy(staging.data() + i * 8, &d, 8);
     }
-    std::memcpy(staging.data() + o_args, b->args.data(), n * sizeof(ShotArgs));
-    e = hipMemcpy(base, staging.data(), o_seeds, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(base + o_seeds, 0, total - o_seeds);
+    e = hipMemcpy(base, staging.data(), total, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void**>(&b->seeds_host), clifft_svm_batch::kSeedSlots * n * sizeof(uint64_t),
                           hipHostMallocDefault);
@@ -1601,8 +1918,13 @@ int clifft_svm_batch_destroy(clifft_svm_batch* b) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->dev) (void)hipFree(b->dev);
-    if (b->items_dev) (void)hipFree(b->items_dev);
-    if (b->gscratch) (void)hipFree(b->gscratch);
+    if (b->plan_dev) (void)hipFree(b->plan_dev);
+    free_group_scratch(b->groups);
+    for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t ev : b->side_done) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : b->t0) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : b->t1) (void)hipEventDestroy(ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->seeds_host) (void)hipHostFree(b->seeds_host);
     for (auto& ev : b->seed_done)
         if (ev) (void)hipEventDestroy(ev);
@@ -1629,28 +1951,73 @@ int clifft_svm_batch_set_tuning(clifft_svm_batch* b, const clifft_svm_batch_tuni
 int clifft_svm_batch_get_tuning(const clifft_svm_batch* b, clifft_svm_batch_tuning* t) {
     if (!b || !t) return fail(CLIFFT_E_INVALID, "null argument");
     *t = b->tuning;
-    t->grid = b->tuning.grid ? b->tuning.grid : b->grid;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_info(const clifft_svm_batch* b, uint32_t max_groups, clifft_svm_batch_group_info* info,
+                          uint32_t* num_groups) {
+    if (!b || !num_groups) return fail(CLIFFT_E_INVALID, "null argument");
+    *num_groups = (uint32_t)b->groups.size();
+    for (uint32_t gi = 0; gi < max_groups && gi < b->groups.size() && info; ++gi) {
+        const BatchGroup& g = b->groups[gi];
+        clifft_svm_batch_group_info& o = info[gi];
+        o.kind = g.L.lane ? 3u : (g.L.global_amps ? 2u : (g.L.block_team ? 1u : 0u));
+        o.programs = (uint32_t)g.progs.size();
+        o.items = (uint32_t)g.items.size();
+        o.grid = g.L.grid;
+        o.block = g.L.block;
+        o.lds_bytes = (uint32_t)g.L.lds_bytes;
+        o.amp_elems = g.L.lane ? 0u : g.amp_elems;
+        o.amp_slices = g.amp_slices;
+        const bool packed = (b->flags & CLIFFT_REC_PACKED) != 0;
+        double bytes = 0.0;
+        for (uint32_t i : g.progs) {
+            const clifft_svm_program* p = b->progs[i];
+            const clifft_svm_records& r = b->outs[i];
+            const double rec = (r.measurements ? row_bytes(p->num_meas, packed) : 0) +
+                               (r.detectors ? row_bytes(p->num_det, packed) : 0) +
+                               (r.observables ? row_bytes(p->num_obs, packed) : 0);
+            bytes += (p->bytes_per_shot + rec) * (double)b->shots[i];
+        }
+        o.algorithmic_bytes = bytes;
+    }
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_time_groups(clifft_svm_batch* b, uint32_t enable) {
+    if (!b) return fail(CLIFFT_E_INVALID, "null batch");
+    b->time_groups = enable != 0;
+    b->timed_valid = false;
+    return CLIFFT_OK;
+}
+
+int clifft_svm_batch_group_times(clifft_svm_batch* b, uint32_t max_groups, float* ms, uint32_t* num_groups) {
+    if (!b || !num_groups) return fail(CLIFFT_E_INVALID, "null argument");
+    if (!b->timed_valid) return fail(CLIFFT_E_INVALID, "no timed eager run since timing was enabled or the plan changed");
+    *num_groups = (uint32_t)b->groups.size();
+    for (uint32_t gi = 0; gi < max_groups && gi < b->groups.size() && ms; ++gi) {
+        ms[gi] = 0.f;
+        if (b->groups[gi].items.empty()) continue;
+        HIP_TRY(hipEventSynchronize(b->t1[gi]));
+        HIP_TRY(hipEventElapsedTime(&ms[gi], b->t0[gi], b->t1[gi]));
+    }
     return CLIFFT_OK;
 }
 
 int clifft_svm_batch_run(clifft_svm_batch* b, const uint64_t* seeds, void* stream) {
     if (!b) return fail(CLIFFT_E_INVALID, "null batch");
-    if (!kPersistentBatchEnabled) return fail(CLIFFT_E_UNSUPPORTED, kPersistentBatchOff);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     int rc = batch_upload_seeds(b, seeds, st);
     if (rc) return rc;
-    if (b->n_items == 0) return CLIFFT_OK;
-    HIP_TRY(launch_batch(b->nw, b->grid, b->block, b->lds_bytes, b->args_dev, b->counter_dev, st));
-    return CLIFFT_OK;
+    return batch_enqueue(b, st);
 }
 
 int clifft_svm_batch_run_graph(clifft_svm_batch* b, const uint64_t* seeds, void* stream) {
     if (!b) return fail(CLIFFT_E_INVALID, "null batch");
-    if (!kPersistentBatchEnabled) return fail(CLIFFT_E_UNSUPPORTED, kPersistentBatchOff);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     int rc = batch_upload_seeds(b, seeds, st);
     if (rc) return rc;
-    if (b->n_items == 0) return CLIFFT_OK;
+    if (b->groups.empty()) return CLIFFT_OK;
     if (!b->graph_exec || b->graph_stream != st || b->graph_epoch != b->epoch) {
         if (b->graph_exec) {
             HIP_TRY(hipStreamSynchronize(st));
@@ -1663,9 +2030,9 @@ int clifft_svm_batch_run_graph(clifft_svm_batch* b, const uint64_t* seeds, void*
         HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            hipError_t le = launch_batch(b->nw, b->grid, b->block, b->lds_bytes, b->args_dev, b->counter_dev, cap);
+            const int qrc = batch_enqueue(b, cap, true);
             e = hipStreamEndCapture(cap, &b->graph);
-            if (le != hipSuccess) e = le;
+            if (qrc != CLIFFT_OK && e == hipSuccess) e = hipErrorUnknown;
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0);
         (void)hipStreamDestroy(cap);
@@ -1679,7 +2046,6 @@ int clifft_svm_batch_run_graph(clifft_svm_batch* b, const uint64_t* seeds, void*
 
 int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stream, clifft_svm_batch_tuning* chosen) {
     if (!b) return fail(CLIFFT_E_INVALID, "null batch");
-    if (!kPersistentBatchEnabled) return fail(CLIFFT_E_UNSUPPORTED, kPersistentBatchOff);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
     if (sample_shots == 0) sample_shots = ~0ull;
     std::vector<uint64_t> seeds(b->progs.size());
@@ -1705,12 +2071,12 @@ int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stre
     clifft_svm_batch_tuning best = default_tuning();
     float best_ms = 0.f;
     int rc = time_of(best, &best_ms);
-    auto try_knob = [&](uint32_t clifft_svm_batch_tuning::*field, std::initializer_list<uint32_t> values) {
+    auto try_u32 = [&](uint32_t clifft_svm_batch_tuning::*field, std::initializer_list<uint32_t> values) {
         for (uint32_t v : values) {
             if (rc) return;
             clifft_svm_batch_tuning t = best;
             t.*field = v;
-            if (t.lds_rank_max < t.wave_rank_max || batch_check_tuning(t) != CLIFFT_OK) continue;
+            if (batch_check_tuning(t) != CLIFFT_OK) continue;
             float ms = 0.f;
             int r = time_of(t, &ms);
             if (r == CLIFFT_E_HIP) {
@@ -1740,24 +2106,21 @@ int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stre
             }
         }
     };
-    if (!rc) try_lane({-1, 2, 4, 6});
-    if (!rc) try_knob(&clifft_svm_batch_tuning::lane_chunk, {256, 512, 2048});
-    if (!rc) try_knob(&clifft_svm_batch_tuning::lds_rank_max, {9, 10, 12, 13});
-    if (!rc) try_knob(&clifft_svm_batch_tuning::wave_rank_max, {5, 6, 8});
-    if (!rc) try_knob(&clifft_svm_batch_tuning::block_threads, {512});
-    if (!rc) try_knob(&clifft_svm_batch_tuning::wave_chunk, {8, 16, 64, 128});
-    if (!rc) try_knob(&clifft_svm_batch_tuning::block_chunk, {2, 4, 16, 32});
-    if (!rc) try_knob(&clifft_svm_batch_tuning::grid, {256, 512, 768});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::concurrent, {0});
+    if (!rc) try_lane({-1, 4, 5});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::lane_chunk, {64, 128, 512, 1024});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::lds_rank_max, {8, 9, 11, 12, 13});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::wave_rank_max, {5, 6, 8});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::global_threads, {512, 1024});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::wave_chunk, {4, 8, 32, 64});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::block_chunk, {1, 2, 8, 16});
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     b->tuning = best;
     rc = batch_plan(b, ~0ull, st);
     if (rc) return rc;
-    if (chosen) {
-        *chosen = best;
-        chosen->grid = b->grid;
-    }
+    if (chosen) *chosen = best;
     return CLIFFT_OK;
 }
 

@@ -36,14 +36,21 @@ hipError_t shot_occupancy(const LaunchShape& L, int* blocks_per_cu);
 // P and A are device memory.
 hipError_t launch_shots(const LaunchShape& L, const DevProgram* P, const ShotArgs* A, hipStream_t st);
 
+// LDS of one workgroup on gfx950.
+constexpr size_t kLdsBytes = 160 * 1024;
+
+// Largest peak rank the executor accepts: amplitude indices are 32-bit.
+constexpr uint32_t kMaxPeakRank = 30;
+
 // Dynamic LDS limit of the batched kernel (160 KiB less its static queue head).
 constexpr size_t kBatchMaxDynLds = 160 * 1024 - 256;
 
-// Batched persistent executor (B and counter are device memory; the counter
-// is cleared by a fill kernel on `st` right before the launch).
-hipError_t batch_occupancy(int nw, uint32_t block, size_t lds_bytes, int* blocks_per_cu);
-hipError_t launch_batch(int nw, uint32_t grid, uint32_t block, size_t lds_bytes, const BatchArgs* B,
-                        uint32_t* counter, hipStream_t st);
+// Batched execution: one persistent launch per launch group (B is device
+// memory; L.grid, L.block and L.lds_bytes are the group's launch shape).
+hipError_t batch_occupancy(const LaunchShape& L, int* blocks_per_cu);
+hipError_t launch_batch_group(const LaunchShape& L, const BatchArgs* B, hipStream_t st);
+// Zero n queue heads with a fill kernel (never a memset: graph-capturable).
+hipError_t clear_counters(uint32_t* counters, uint32_t n, hipStream_t st);
 
 // Adjoint-gradient executor (P and A are device memory).
 hipError_t adjoint_occupancy(bool global_amps, uint32_t block, size_t lds_bytes, int* blocks_per_cu);

@@ -235,20 +235,22 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
 }
 
 // Lane tier: one shot per lane.  Wavefront `wave` (0 <= wave < waves in the
-// workgroup) of this workgroup owns an LDS slice of lane_wave_bytes(); the
-// calling wavefront runs shot groups g = g_first, g_first + g_step, ...,
-// lane l taking shot s_lo + 64 g + l while that is below s_hi.  FW is the
-// frame word (uint32_t for programs of at most 32 qubits).
+// workgroup) of this workgroup owns the LDS slice [wave * wave_stride, +
+// lane_wave_bytes()) (wave_stride = lane_wave_bytes() of the program, or
+// the largest of a batched launch's programs); the calling wavefront runs
+// shot groups g = g_first, g_first + g_step, ..., lane l taking shot
+// s_lo + 64 g + l while that is below s_hi.  FW is the frame word
+// (uint32_t for programs of at most 32 qubits).
 template <bool kFrameFirst, class FW>
 __device__ __forceinline__ void lane_range(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint64_t g_first, uint64_t g_step,
-                                           uint64_t s_lo, uint64_t s_hi, uint64_t seed) {
+                                           uint64_t s_lo, uint64_t s_hi, uint64_t seed, uint32_t wave_stride) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = uni(threadIdx.x >> 6);
     cptr<DevProgram> P = opaque(Pg);
     const uint32_t peak = P->peak_rank, slots = P->total_meas_slots, ndet = P->num_det, nobs = P->num_obs;
     const uint32_t wm = lane_words(slots), wd = lane_words(ndet), nwords = wm + wd + lane_words(nobs);
-    unsigned char* base = smem + (size_t)wave * lane_wave_bytes(peak, slots, ndet, nobs);
+    unsigned char* base = smem + (size_t)wave * wave_stride;
     LaneInterp<FW> S;
     S.amp = (lds_col_d2*)(base) + lane;
     lds_col_u32* recs = (lds_col_u32*)(base + (64u * 16u << peak)) + lane;
@@ -332,7 +334,9 @@ __global__ void __launch_bounds__(256, MINB) lane_kernel(const DevProgram* P, co
     cptr<ShotArgs> Ag = cp(A);
     const uint64_t wpb = blockDim.x >> 6;
     const uint64_t wave_global = (uint64_t)blockIdx.x * wpb + uni(threadIdx.x >> 6);
-    lane_range<MINB == 4, FW>(cp(P), Ag, wave_global, (uint64_t)gridDim.x * wpb, 0, Ag->shot_count, Ag->seed);
+    cptr<DevProgram> Pc = cp(P);
+    const uint32_t stride = lane_wave_bytes(Pc->peak_rank, Pc->total_meas_slots, Pc->num_det, Pc->num_obs);
+    lane_range<MINB == 4, FW>(Pc, Ag, wave_global, (uint64_t)gridDim.x * wpb, 0, Ag->shot_count, Ag->seed, stride);
 }
 
 // One program: teams stride over the launch's shots.
@@ -347,56 +351,62 @@ __global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram*
     shot_range<FK, BT, GA>(cp(P), Ag, team_global, Ag->shot_count, nteams, gamp, Ag->seed);
 }
 
-// One work item of the batched executor, kept out of line: the caller's
-// queue loop and the three team shapes then each get their own register
-// allocation (the save/restore at the call happens once per item).
-__device__ __noinline__ void batch_item_lane(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint32_t lo, uint32_t hi,
-                                            uint64_t seed) {
-    lane_range<false, uint64_t>(Pg, Ag, uni(threadIdx.x >> 6), blockDim.x >> 6, lo, hi, seed);
-}
-
-template <int FK, bool BT, bool GA>
-__device__ __noinline__ void batch_item(cptr<DevProgram> Pg, cptr<ShotArgs> Ag, uint32_t lo, uint32_t hi,
-                                        uint32_t step, dv2* gamp, uint64_t seed) {
-    shot_range<FK, BT, GA>(Pg, Ag, lo, hi, step, gamp, seed);
-}
-
-// Many programs in one persistent launch.  Each workgroup pops work items
+// ---------------------------------------------------------------------------
+// Batched execution: many programs per launch.  A batch is split into
+// launch groups, one per kernel shape (the shapes of the per-program
+// launches); each group's launch is persistent and its teams pop work items
 // (a shot range of one program, in decreasing order of estimated cost) from
-// a device queue head until the queue is empty; the item's tier picks the
-// team shape: wavefront teams with LDS amplitudes, one workgroup team with
-// LDS amplitudes, or one workgroup team with amplitudes in the workgroup's
-// global slice.  Every workgroup leaves the loop once the head passes the
-// item count, so the grid always drains.
-template <int FK>
-__global__ void __launch_bounds__(512, 4) batch_kernel(const BatchArgs* B) {
-    __shared__ uint32_t head;
+// the group's device queue until it is empty.  The per-program ShotArgs of a
+// group carry the group's largest amplitude count and record stride, so
+// every team of a workgroup carves the group's LDS the same way whichever
+// program it runs.
+// ---------------------------------------------------------------------------
+
+// One wavefront pops the next item index (lane 0's atomic, broadcast).
+CSVM_INLINE uint32_t pop_wave(uint32_t* counter) {
+    uint32_t v = 0;
+    if ((threadIdx.x & 63u) == 0) v = atomicAdd(counter, 1u);
+    return uni((uint32_t)__shfl((int)v, 0, 64));
+}
+
+template <int MINB, class FW>
+__global__ void __launch_bounds__(256, MINB) batch_lane_kernel(const BatchArgs* B) {
     cptr<BatchArgs> Bg = cp(B);
     for (;;) {
-        if (threadIdx.x == 0) head = atomicAdd(Bg->counter, 1u);
-        __syncthreads();
-        const uint32_t it = uni(head);
-        __syncthreads();
+        const uint32_t it = pop_wave(Bg->counter);
         cptr<BatchArgs> Bc = opaque(Bg);
         if (it >= Bc->n_items) break;
         cptr<BatchItem> I = cp(Bc->items) + it;
-        const uint32_t pid = I->prog, tier = I->tier;
-        const uint32_t lo = I->shot_lo, hi = I->shot_hi;
-        cptr<DevProgram> Pg = cp(cp(Bc->progs)[pid]);
-        cptr<ShotArgs> Ag = cp(Bc->args) + pid;
-        const uint64_t seed = cp(Bc->seeds)[pid];
-        if (tier == TIER_LANE) {
-            batch_item_lane(Pg, Ag, lo, hi, seed);
-        } else if (tier == TIER_WAVE) {
-            const uint32_t w = uni(threadIdx.x >> 6);
-            batch_item<FK, false, false>(Pg, Ag, lo + w, hi, blockDim.x >> 6, nullptr, seed);
-        } else if (tier == TIER_LDS) {
-            batch_item<FK, true, false>(Pg, Ag, lo, hi, 1, nullptr, seed);
+        const uint32_t pid = I->prog;
+        lane_range<MINB == 4, FW>(cp(cp(Bc->progs)[pid]), cp(Bc->args) + pid, 0, 1, I->shot_lo, I->shot_hi,
+                                  cp(Bc->seeds)[pid], Bc->wave_stride);
+    }
+}
+
+template <int FK, bool BT, bool GA>
+__global__ void __launch_bounds__(BT ? 1024 : 256) batch_team_kernel(const BatchArgs* B) {
+    __shared__ uint32_t head;
+    cptr<BatchArgs> Bg = cp(B);
+    const uint32_t tpb = BT ? 1u : (blockDim.x >> 6);
+    const uint32_t team_in_block = BT ? 0u : uni(threadIdx.x >> 6);
+    const uint64_t team_global = (uint64_t)blockIdx.x * tpb + team_in_block;
+    dv2* gamp = GA ? Bg->gscratch + team_global * Bg->team_elems : nullptr;
+    for (;;) {
+        uint32_t it;
+        if (BT) {
+            if (threadIdx.x == 0) head = atomicAdd(Bg->counter, 1u);
+            __syncthreads();
+            it = uni(head);
+            __syncthreads();
         } else {
-            dv2* gamp = Bc->gscratch + (uint64_t)blockIdx.x * Bc->gscratch_elems;
-            batch_item<FK, true, true>(Pg, Ag, lo, hi, 1, gamp, seed);
+            it = pop_wave(Bg->counter);
         }
-        __syncthreads();
+        cptr<BatchArgs> Bc = opaque(Bg);
+        if (it >= Bc->n_items) break;
+        cptr<BatchItem> I = cp(Bc->items) + it;
+        const uint32_t pid = I->prog;
+        shot_range<FK, BT, GA>(cp(cp(Bc->progs)[pid]), cp(Bc->args) + pid, I->shot_lo, I->shot_hi, 1, gamp,
+                               cp(Bc->seeds)[pid]);
     }
 }
 
@@ -542,39 +552,45 @@ hipError_t launch_shots(const LaunchShape& L, const DevProgram* P, const ShotArg
 }
 
 template <int FK>
-static const void* batch_fn() {
-    return reinterpret_cast<const void*>(batch_kernel<FK>);
+static const void* batch_team_fn(bool block_team, bool global_amps) {
+    if (block_team)
+        return global_amps ? reinterpret_cast<const void*>(batch_team_kernel<FK, true, true>)
+                           : reinterpret_cast<const void*>(batch_team_kernel<FK, true, false>);
+    return global_amps ? reinterpret_cast<const void*>(batch_team_kernel<FK, false, true>)
+                       : reinterpret_cast<const void*>(batch_team_kernel<FK, false, false>);
 }
 
-hipError_t batch_prepare(int nw) {
-    static std::atomic<bool> done[2] = {{false}, {false}};
-    const void* fn = nw == 1 ? batch_fn<FRAME_W1>() : batch_fn<FRAME_LANES>();
-    if (done[nw == 1].load(std::memory_order_acquire)) return hipSuccess;
-    // the queue head is 16 B of static LDS; dynamic LDS may use the rest
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kBatchMaxDynLds);
-    if (e == hipSuccess) done[nw == 1].store(true, std::memory_order_release);
-    return e;
+static const void* batch_fn(const LaunchShape& L) {
+    if (L.lane) {
+        if (L.lane_narrow)
+            return L.lane_dense ? reinterpret_cast<const void*>(batch_lane_kernel<6, uint32_t>)
+                                : reinterpret_cast<const void*>(batch_lane_kernel<4, uint32_t>);
+        return L.lane_dense ? reinterpret_cast<const void*>(batch_lane_kernel<6, uint64_t>)
+                            : reinterpret_cast<const void*>(batch_lane_kernel<4, uint64_t>);
+    }
+    return L.nw == 1 ? batch_team_fn<FRAME_W1>(L.block_team, L.global_amps)
+                     : batch_team_fn<FRAME_LANES>(L.block_team, L.global_amps);
 }
 
-hipError_t batch_occupancy(int nw, uint32_t block, size_t lds_bytes, int* blocks_per_cu) {
-    hipError_t e = batch_prepare(nw);
+hipError_t batch_occupancy(const LaunchShape& L, int* blocks_per_cu) {
+    const void* fn = batch_fn(L);
+    // dynamic LDS may use what the static queue head leaves
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBatchMaxDynLds);
     if (e != hipSuccess) return e;
-    const void* fn = nw == 1 ? batch_fn<FRAME_W1>() : batch_fn<FRAME_LANES>();
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, (int)block, lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, (int)L.block, L.lds_bytes);
 }
 
-hipError_t launch_batch(int nw, uint32_t grid, uint32_t block, size_t lds_bytes, const BatchArgs* B,
-                        uint32_t* counter, hipStream_t st) {
-    hipError_t e = batch_prepare(nw);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(clear_u32_kernel, dim3(1), dim3(64), 0, st, counter, 1u);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (nw == 1)
-        hipLaunchKernelGGL(batch_kernel<FRAME_W1>, dim3(grid), dim3(block), lds_bytes, st, B);
-    else
-        hipLaunchKernelGGL(batch_kernel<FRAME_LANES>, dim3(grid), dim3(block), lds_bytes, st, B);
+hipError_t clear_counters(uint32_t* counters, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(clear_u32_kernel, dim3((n + 63) / 64), dim3(64), 0, st, counters, n);
     return hipGetLastError();
+}
+
+// The dynamic-LDS attribute was set by batch_occupancy when the plan was
+// made, so a launch is a stream operation only (graph-capturable).
+hipError_t launch_batch_group(const LaunchShape& L, const BatchArgs* B, hipStream_t st) {
+    const void* fn = batch_fn(L);
+    void* args[] = {&B};
+    return hipLaunchKernel(fn, dim3(L.grid), dim3(L.block), args, L.lds_bytes, st);
 }
 
 }  // namespace csvm

@@ -80,12 +80,16 @@ def _run_torch(adj: Adjoint, thetas, coeffs, want_grad: bool):
     cur = torch.cuda.current_stream(thetas.device)
     side = torch.cuda.Stream(device=thetas.device)
     side.wait_stream(cur)
+    if thetas.dim() != 2 or thetas.shape[1] != adj.num_params:
+        raise ValueError(f"thetas must have shape [batch, {adj.num_params}], got {tuple(thetas.shape)}")
+    if coeffs.numel() != adj.num_exp_vals:
+        raise ValueError(f"expected {adj.num_exp_vals} coefficients, got {coeffs.numel()}")
     with torch.cuda.stream(side):
         th = thetas.detach().to(torch.float64).contiguous()
         B = th.shape[0]
         ev = torch.empty((B, adj.num_exp_vals), dtype=torch.float64, device=th.device)
         g = torch.empty((B, adj.num_params), dtype=torch.float64, device=th.device) if want_grad else None
-        c = coeffs.detach().to(device=th.device, dtype=torch.float64).contiguous()
+        c = coeffs.detach().to(device=th.device, dtype=torch.float64).reshape(-1).contiguous()
         _lib.check(_lib.lib().clifft_svm_adjoint_run(adj.handle, th.data_ptr() if th.numel() else None, B,
                                                      c.data_ptr() if c.numel() else None,
                                                      ev.data_ptr() if ev.numel() else None,
@@ -102,21 +106,25 @@ def _function():
     import torch
 
     class ExpectationFunction(torch.autograd.Function):
-        """F[b] = sum_j coeffs[j] <P_j>(thetas[b]); backward multiplies the
-        adjoint gradient computed in the same kernel by the incoming grad."""
+        """F[b] = sum_j coeffs[j] <P_j>(thetas[b]).  dF/dthetas is the
+        adjoint gradient computed in the same kernel; dF/dcoeffs[j] is
+        <P_j>(thetas[b]) itself.  backward multiplies both by the incoming
+        grad."""
 
         @staticmethod
         def forward(ctx, thetas, adj, coeffs):
             want = thetas.requires_grad
             ev, g, c = _run_torch(adj, thetas, coeffs, want)
-            if want:
-                ctx.save_for_backward(g)
+            ctx.save_for_backward(g if want else None, ev if coeffs.requires_grad else None)
+            ctx.coeffs_shape = coeffs.shape
             return ev @ c
 
         @staticmethod
         def backward(ctx, grad_out):
-            (g,) = ctx.saved_tensors
-            return grad_out.unsqueeze(1) * g, None, None
+            g, ev = ctx.saved_tensors
+            d_th = grad_out.unsqueeze(1) * g if ctx.needs_input_grad[0] else None
+            d_c = (grad_out @ ev).reshape(ctx.coeffs_shape) if ctx.needs_input_grad[2] else None
+            return d_th, None, d_c
 
     return ExpectationFunction
 

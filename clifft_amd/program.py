@@ -222,6 +222,9 @@ def make_exp_val(cp_idx, ev_idx):
 
 
 # --------------------------------------------------------------------------
+# CLIFFT_SVM_ABI_VERSION of include/clifft_svm.h
+ABI_VERSION = 2
+
 # ctypes mirror of clifft_program_view
 # --------------------------------------------------------------------------
 _P = ctypes.c_void_p
@@ -381,7 +384,7 @@ class Program:
     def view(self) -> ProgramView:
         """The C-ABI view; arrays stay owned by this object."""
         v = ProgramView()
-        v.abi_version = 1
+        v.abi_version = ABI_VERSION
         v.num_qubits = self.num_qubits
         v.peak_rank = self.peak_rank
         v.num_measurements = self.num_measurements
@@ -443,7 +446,8 @@ class Program:
         return p
 
     def copy(self) -> "Program":
-        d = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.__dict__.items() if k != "_view"}
+        d = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.__dict__.items()
+             if k not in ("_view", "_device_program")}
         return Program(**d)
 
     # ------------------------------------------------------------ blob I/O

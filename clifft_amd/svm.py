@@ -30,16 +30,27 @@ from .sample_result import SampleResult
 
 
 class DeviceProgram:
-    """A Program uploaded to the GPU (bytecode + constant pool + plan)."""
+    """A Program uploaded to the GPU (bytecode + constant pool + plan).
 
-    def __init__(self, program: Program):
-        self.program = program
+    Created directly, it keeps its Program alive.  The copy that
+    device_program() caches rides on the Program object and refers back to
+    it weakly, so dropping the Program frees its device state."""
+
+    def __init__(self, program: Program, _weak: bool = False):
+        self._program = weakref.ref(program) if _weak else (lambda p=program: p)
         L = _lib.lib()
         h = ctypes.c_void_p()
         view = program.view()
         _lib.check(L.clifft_svm_program_create(ctypes.byref(view), ctypes.byref(h)), "program_create")
         self.handle = h
         self._fin = weakref.finalize(self, L.clifft_svm_program_destroy, h)
+
+    @property
+    def program(self) -> Program:
+        p = self._program()
+        if p is None:
+            raise RuntimeError("the Program of this DeviceProgram no longer exists")
+        return p
 
     def info(self) -> dict:
         info = _lib.ProgramInfo()
@@ -50,18 +61,15 @@ class DeviceProgram:
         self._fin()
 
 
-_CACHE: dict = {}
-
-
 def device_program(program) -> DeviceProgram:
+    """The program's device upload, made on first use and kept on the
+    Program object (freed with it)."""
     if isinstance(program, DeviceProgram):
         return program
-    key = id(program)
-    ent = _CACHE.get(key)
-    if ent is not None and ent[0]() is program:
-        return ent[1]
-    dp = DeviceProgram(program)
-    _CACHE[key] = (weakref.ref(program), dp)
+    dp = program.__dict__.get("_device_program")
+    if dp is None:
+        dp = DeviceProgram(program, _weak=True)
+        program._device_program = dp
     return dp
 
 

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CLIFFT_SVM_ABI_VERSION 1u
+#define CLIFFT_SVM_ABI_VERSION 2u
 
 /* ---------------------------------------------------------------------------
  * Opcodes: numbering is the reference enum order (backend.h:25-86).
@@ -353,13 +353,16 @@ int clifft_svm_execute(clifft_svm_program* prog, clifft_svm_state* state);
 /* --- batching ---------------------------------------------------------------*/
 /* No counterpart in the reference (its VM runs one program per call,
  * src/clifft/svm/svm.h:302-306).  Many programs of uneven peak rank and
- * bytecode length run in ONE persistent launch: work items (a shot range of
- * one program) are sorted by descending estimated cost and drained by the
- * workgroups from a device queue.  Each item runs with the team shape of
- * its tier: one shot per lane (peak_rank <= lane_rank_max, at most 64
- * qubits), one shot per wavefront (peak_rank <= wave_rank_max), one shot per
- * workgroup with LDS-resident amplitudes (<= lds_rank_max), or one shot per
- * workgroup with amplitudes in a per-workgroup global slice.
+ * bytecode length run as a few persistent launches, one per launch group:
+ * the programs that share a kernel shape -- one shot per lane (peak_rank <=
+ * lane_rank_max, at most 64 qubits; split by frame-word width), one shot
+ * per wavefront (peak_rank <= wave_rank_max), one shot per workgroup with
+ * LDS-resident amplitudes (<= lds_rank_max; split by team size), or one
+ * shot per workgroup with amplitudes in a global slice (global_threads per
+ * team).  A group's work items (a shot range of one program) are sorted by
+ * descending estimated cost and drained by its teams from a device queue.
+ * The groups run one after another on the caller's stream, or side by side
+ * on the batch's own streams (concurrent = 1).
  *
  * Shot s of program i draws from the per-shot stream of (seeds[i], s), as in
  * clifft_svm_sample, so the records equal that call's; the team shape (and
@@ -370,16 +373,30 @@ int clifft_svm_execute(clifft_svm_program* prog, clifft_svm_state* state);
 typedef struct clifft_svm_batch clifft_svm_batch;
 
 typedef struct {
-    uint32_t block_threads; /* 256 or 512 */
-    uint32_t wave_rank_max; /* <= 8 */
-    uint32_t lds_rank_max;  /* <= 13, >= wave_rank_max */
-    uint32_t wave_chunk;    /* shots per work item, wavefront tier */
-    uint32_t block_chunk;   /* shots per work item, workgroup tiers */
-    uint32_t grid;          /* workgroups (0 = resident capacity of the device) */
-    int32_t lane_rank_max;  /* <= 64-qubit programs up to this rank run one shot
-                               per lane (-1 = never), <= 6 */
-    uint32_t lane_chunk;    /* shots per work item, lane tier */
+    uint32_t global_threads; /* 256, 512 or 1024: team size of the global-amplitude group */
+    uint32_t wave_rank_max;  /* <= 8 */
+    uint32_t lds_rank_max;   /* <= 13, >= wave_rank_max */
+    uint32_t wave_chunk;     /* shots per work item, wavefront teams */
+    uint32_t block_chunk;    /* shots per work item, workgroup teams */
+    uint32_t concurrent;     /* 1: launch groups on parallel streams; 0: in sequence */
+    int32_t lane_rank_max;   /* <= 64-qubit programs up to this rank run one shot
+                                per lane (-1 = never), <= 6 */
+    uint32_t lane_chunk;     /* shots per work item, lane tier (a multiple of 64) */
 } clifft_svm_batch_tuning;
+
+/* One launch group of a batch's current plan. */
+typedef struct {
+    uint32_t kind;       /* 0 wavefront teams, 1 LDS workgroup teams, 2 global workgroup teams, 3 lanes */
+    uint32_t programs;   /* member programs */
+    uint32_t items;      /* work items in its queue */
+    uint32_t grid;       /* persistent workgroups */
+    uint32_t block;      /* threads per workgroup */
+    uint32_t lds_bytes;  /* dynamic LDS per workgroup */
+    uint32_t amp_elems;  /* amplitudes per team slice (team kinds) */
+    uint32_t amp_slices; /* slices per team (global kind) */
+    double algorithmic_bytes; /* per launch: the members' per-shot state-traffic
+                                 model times their shots, plus record bytes written */
+} clifft_svm_batch_group_info;
 
 /* outs[i] are device pointers; CLIFFT_REC_PACKED is honoured. */
 int clifft_svm_batch_create(clifft_svm_program* const* programs, const uint64_t* shots,
@@ -388,21 +405,27 @@ int clifft_svm_batch_create(clifft_svm_program* const* programs, const uint64_t*
 int clifft_svm_batch_destroy(clifft_svm_batch* batch);
 int clifft_svm_batch_set_tuning(clifft_svm_batch* batch, const clifft_svm_batch_tuning* tuning);
 int clifft_svm_batch_get_tuning(const clifft_svm_batch* batch, clifft_svm_batch_tuning* tuning);
-/* In this build the persistent launch is switched off: clifft_svm_batch_tune,
- * clifft_svm_batch_run, clifft_svm_batch_run_graph and clifft_svm_batch_sample
- * without CLIFFT_BATCH_PER_PROGRAM return CLIFFT_E_UNSUPPORTED (its first device
- * run ended in a GPU memory fault whose cause has not been found).
- *
- * Autotuner: times candidate tunings on the batch itself (each program
+/* The current plan: *num_groups groups, the first max_groups of them
+ * described in info (may be NULL). */
+int clifft_svm_batch_info(const clifft_svm_batch* batch, uint32_t max_groups, clifft_svm_batch_group_info* info,
+                          uint32_t* num_groups);
+/* Per-group kernel timing of eager runs: with timing enabled, each group's
+ * launch in clifft_svm_batch_run is bracketed by timing events on the stream
+ * it goes to; clifft_svm_batch_group_times waits for the last such run and
+ * returns each group's kernel time in ms (graph replays are not timed). */
+int clifft_svm_batch_time_groups(clifft_svm_batch* batch, uint32_t enable);
+int clifft_svm_batch_group_times(clifft_svm_batch* batch, uint32_t max_groups, float* ms, uint32_t* num_groups);
+/* Autotuner: times candidate tunings on the batch itself (each program
  * capped at `sample_shots` shots, outputs overwritten) with HIP events on
  * `stream`, keeps the fastest and returns it in `chosen` (may be NULL). */
 int clifft_svm_batch_tune(clifft_svm_batch* batch, uint64_t sample_shots, void* stream,
                           clifft_svm_batch_tuning* chosen);
-/* One persistent launch with per-program seeds (host array).  Asynchronous
+/* Every group's launch with per-program seeds (host array).  Asynchronous
  * on `stream` (NULL = the library stream). */
 int clifft_svm_batch_run(clifft_svm_batch* batch, const uint64_t* seeds, void* stream);
-/* Same work through a captured HIP graph (queue reset + persistent launch),
- * captured on first use per stream and tuning and replayed afterwards. */
+/* Same work through a captured HIP graph (queue resets + the group
+ * launches), captured on first use per stream and plan and replayed
+ * afterwards. */
 int clifft_svm_batch_run_graph(clifft_svm_batch* batch, const uint64_t* seeds, void* stream);
 
 /* --- differentiation ---------------------------------------------------------*/

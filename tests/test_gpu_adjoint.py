@@ -92,6 +92,22 @@ def test_autograd_function(mods):
     np.testing.assert_allclose(th.grad.cpu().numpy(), g_ref * w.cpu().numpy()[:, None], atol=1e-13, rtol=0)
     th2 = torch.tensor(th_np[:1], dtype=torch.float64, device="cuda:0", requires_grad=True)
     assert torch.autograd.gradcheck(lambda t: grad.expectation(adj, t, c), (th2,), eps=1e-6, atol=1e-6)
+    # dF/dcoeffs = exp_vals, alone and together with dF/dthetas
+    c2 = torch.tensor(coeffs, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    th3 = torch.tensor(th_np, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    (grad.expectation(adj, th3, c2) * w).sum().backward()
+    np.testing.assert_allclose(c2.grad.cpu().numpy(), w.cpu().numpy() @ ev_ref, atol=1e-13, rtol=0)
+    np.testing.assert_allclose(th3.grad.cpu().numpy(), g_ref * w.cpu().numpy()[:, None], atol=1e-13, rtol=0)
+    c3 = torch.tensor(coeffs, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    assert torch.autograd.gradcheck(lambda cc_: grad.expectation(adj, th2.detach(), cc_), (c3,), eps=1e-6,
+                                    atol=1e-8)
+    # shapes the kernel would read past are refused before any launch
+    with pytest.raises(ValueError):
+        grad.expectation(adj, th[0].detach(), c)
+    with pytest.raises(ValueError):
+        grad.expectation(adj, th[:, :5].detach(), c)
+    with pytest.raises(ValueError):
+        grad.expectation(adj, th.detach(), c[:-1])
 
 
 def test_workload_rz24_matches_oracle(mods, orc):

@@ -1,9 +1,11 @@
-"""Batched persistent executor vs per-program launches and the oracle.
+"""Batched execution vs per-program launches and the oracle.
 
 Every program of a batch must produce the records `svm.sample(program,
 shots, seed)` produces (the per-shot stream of (seed, shot) is the same; the
 team shape, and so the reduction order, can differ between the two paths,
-which moves expectation values by rounding only).
+which moves expectation values by rounding only).  The batch runs one
+persistent launch per launch group (programs sharing a kernel shape), each
+draining its own device queue.
 """
 
 from __future__ import annotations
@@ -15,14 +17,7 @@ import pytest
 
 from randprog import random_program
 
-# The first device run of this module (its first test, four seconds in)
-# ended in a GPU memory fault.  The kernels it ran are byte-identical to the
-# ones that passed this module before; a CPU emulation of the same programs
-# (every kernel under AddressSanitizer, lock-step checks of the uniform
-# broadcasts) found no out-of-bounds access.  With no cause shown, the module
-# stays off the device.
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="GPU memory fault on its first device run, cause not found; kept off the device")]
+pytestmark = pytest.mark.gpu
 
 ATOL = 1e-12
 
@@ -113,12 +108,12 @@ def _snapshot(bufs):
     return [tuple(t.cpu().numpy().copy() for t in b) for b in bufs]
 
 
-TUNINGS = [dict(block_threads=512), dict(lds_rank_max=9), dict(lds_rank_max=13), dict(wave_rank_max=5),
-           dict(wave_rank_max=8, lds_rank_max=8), dict(wave_chunk=4, block_chunk=1), dict(grid=7),
-           dict(block_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32),
+TUNINGS = [dict(concurrent=0), dict(global_threads=512), dict(global_threads=1024), dict(lds_rank_max=8),
+           dict(lds_rank_max=13), dict(wave_rank_max=5), dict(wave_rank_max=8, lds_rank_max=8),
+           dict(wave_chunk=1, block_chunk=1), dict(wave_chunk=128, block_chunk=64),
+           dict(global_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32),
            dict(lane_rank_max=-1), dict(lane_rank_max=5, lane_chunk=100), dict(lane_rank_max=1, lane_chunk=64),
-           dict(lane_rank_max=6, lane_chunk=256),
-           dict(block_threads=512, lane_rank_max=4, lane_chunk=3000)]
+           dict(lane_rank_max=6, lane_chunk=1024), dict(concurrent=0, lane_rank_max=4, lane_chunk=3000)]
 
 
 def test_batch_tunings_agree(mods):
@@ -153,8 +148,9 @@ def test_batch_autotune(mods):
     progs, shots, seeds = mixed_programs(31, 33)
     b, bufs = _device_batch(batch, progs, shots, packed=True)
     chosen = b.tune(sample_shots=16)
-    assert chosen["block_threads"] in (256, 512) and chosen["grid"] > 0
+    assert chosen["global_threads"] in (256, 512, 1024) and chosen["concurrent"] in (0, 1)
     assert chosen["wave_rank_max"] <= chosen["lds_rank_max"] <= 13
+    assert b.tuning == chosen
     b.run(seeds)
     torch.cuda.synchronize()
     got = _snapshot(bufs)
@@ -166,14 +162,17 @@ def test_batch_autotune(mods):
     b.close()
 
 
-def test_batch_graph_replays(mods):
-    """Captured graph replays, with eager work and synchronizations between
-    them, each match an eager run with the same seeds."""
+@pytest.mark.parametrize("concurrent", [1, 0])
+def test_batch_graph_replays(mods, concurrent):
+    """Captured graph replays (launch groups on parallel streams, or in
+    sequence), with eager work and synchronizations between them, each match
+    an eager run with the same seeds."""
     import torch
     svm, batch = mods
-    announce("batch-graph")
+    announce("batch-graph", concurrent)
     progs, shots, seeds = mixed_programs(41, 15)
     b, bufs = _device_batch(batch, progs, shots)
+    b.set_tuning(concurrent=concurrent)
     rng = np.random.default_rng(3)
     for rep in range(4):
         sd = [int(x) for x in rng.integers(0, 2**63, size=len(progs))]
@@ -195,13 +194,50 @@ def test_batch_graph_replays(mods):
     b.close()
 
 
+def test_batch_groups_cover_every_shape(mods):
+    """The mixed batch plans one launch group per kernel shape: lanes (by
+    frame width), wavefront teams, LDS workgroup teams by size, global
+    teams; every program sits in exactly one group."""
+    svm, batch = mods
+    progs, shots, seeds = mixed_programs(61, 26, wide=True)
+    b, bufs = _device_batch(batch, progs, shots)
+    groups = b.groups()
+    assert {g["kind"] for g in groups} == {"lane", "wave", "lds", "global"}
+    assert sum(g["programs"] for g in groups) == len(progs)
+    for g in groups:
+        assert 0 < g["lds_bytes"] <= 160 * 1024 - 256 and g["grid"] > 0 and g["items"] > 0
+        if g["kind"] == "global":
+            assert g["block"] in (256, 512, 1024) and g["amp_slices"] in (1, 2)
+    b.close()
+
+
+def test_batch_c3_programs_match_oracle(mods, orc):
+    """64 programs of the 4096-program workload generator (16-40 qubits,
+    T-count = peak rank 4-16, 40 layers, final M on every qubit) in one
+    batch, 24 shots each: records bit-exact against the oracle's per-shot
+    streams."""
+    from clifft_amd import workloads as W
+    from clifft_amd.compiler import compile as ccompile
+    svm, batch = mods
+    announce("batch-c3")
+    progs = [ccompile(text) for text, _ in W.batch_programs(count=64, seed=42, shots=24)]
+    assert min(p.peak_rank for p in progs) <= 6 and max(p.peak_rank for p in progs) >= 14
+    seeds = [42 ^ i for i in range(len(progs))]
+    got = batch.batch_sample(progs, 24, seeds, packed=True)
+    for i, p in enumerate(progs):
+        m, d, o, e, _ = orc.sample(p, 24, seed=seeds[i], per_shot=True)
+        assert np.array_equal(got[i].measurements, np.packbits(m, axis=1, bitorder="little")), (i, p.peak_rank)
+
+
 def test_batch_rejects_bad_tuning(mods):
     svm, batch = mods
     from clifft_amd._lib import SvmError
     progs, shots, seeds = mixed_programs(51, 3)
     b, bufs = _device_batch(batch, progs, shots)
     with pytest.raises(SvmError):
-        b.set_tuning(block_threads=300)
+        b.set_tuning(global_threads=300)
+    with pytest.raises(SvmError):
+        b.set_tuning(concurrent=2)
     with pytest.raises(SvmError):
         b.set_tuning(wave_rank_max=9)
     with pytest.raises(SvmError):

@@ -54,3 +54,51 @@ def test_valid_program_passes_validation():
     rc, msg = create(p)
     # with no GPU the call reaches device selection and reports that instead
     assert rc in (0, L.E_NODEVICE), (rc, msg)
+
+
+def test_rejects_peak_rank_beyond_32bit_indices():
+    # 32-bit amplitude indices: peak 30 is the largest accepted rank
+    for peak, ok in ((30, True), (31, False), (40, False)):
+        p = build_program([make_expand(0), make_array_h(0)], peak_rank=peak, num_qubits=48)
+        rc, msg = create(p)
+        if ok:
+            assert rc in (0, L.E_NODEVICE), (rc, msg)
+        else:
+            assert rc == L.E_UNSUPPORTED and "peak_rank <= 30" in msg, (rc, msg)
+
+
+def test_device_program_freed_with_program(monkeypatch):
+    """device_program() keeps its upload on the Program and refers back to
+    it weakly: dropping the Program destroys the device handle."""
+    import gc
+
+    from clifft_amd import svm
+
+    created, destroyed = [], []
+
+    class FakeLib:
+        def clifft_svm_program_create(self, view, out):
+            out._obj.value = 1000 + len(created)
+            created.append(out._obj.value)
+            return 0
+
+        def clifft_svm_program_destroy(self, h):
+            destroyed.append(h.value)
+            return 0
+
+    fake = FakeLib()
+    monkeypatch.setattr(svm._lib, "lib", lambda: fake)
+    p = build_program([make_expand(0), make_array_h(0)], peak_rank=1, num_qubits=2)
+    dp = svm.device_program(p)
+    assert svm.device_program(p) is dp and dp.program is p and created == [1000]
+    del dp
+    assert destroyed == []  # still reachable through the program
+    del p
+    gc.collect()
+    assert destroyed == [1000]
+    # a DeviceProgram made directly keeps its program alive
+    q = build_program([make_expand(0)], peak_rank=1, num_qubits=2)
+    d2 = svm.DeviceProgram(q)
+    del q
+    gc.collect()
+    assert d2.program.peak_rank == 1 and destroyed == [1000]
