@@ -77,6 +77,8 @@ SIGNATURES = {
     "clifft_svm_program_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ProgramInfo)]),
     "clifft_svm_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                          ctypes.POINTER(Records), ctypes.c_void_p]),
+    "clifft_svm_sample_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                                 ctypes.POINTER(Records), ctypes.c_void_p]),
     "clifft_svm_sample_survivors": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                                    ctypes.c_uint32, ctypes.POINTER(Survivors),
                                                    ctypes.POINTER(Records), ctypes.c_void_p]),

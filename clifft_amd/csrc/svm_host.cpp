@@ -1052,6 +1052,100 @@ int clifft_svm_sample(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Sampling that allocates nothing
+// ---------------------------------------------------------------------------
+// The global-memory amplitudes of clifft_svm_sample_noalloc live in this
+// array: 2^22 complex128 = 64 MiB, one copy per device, held until the
+// process ends.  In host code its name is only a stand-in; the device
+// address comes from hipGetSymbolAddress with the device current, once per
+// device, and only that address is passed to kernels.
+static constexpr size_t kNoAllocScratchElems = (size_t)1 << 22;
+static __device__ __align__(256) dv2 scratch[kNoAllocScratchElems];
+
+namespace {
+
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[64] = {};
+
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(CLIFFT_E_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(CLIFFT_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return CLIFFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clifft_svm_sample_noalloc(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags,
+                              const clifft_svm_records* out, void* stream) {
+    if (!p) return fail(CLIFFT_E_INVALID, "null program");
+    if (!(flags & CLIFFT_REC_DEVICE))
+        return fail(CLIFFT_E_INVALID, "sample_noalloc writes device outputs only (CLIFFT_REC_DEVICE)");
+    if (p->has_forced_ops)
+        return fail(CLIFFT_E_INVALID, "program contains *_FORCED opcodes; use clifft_svm_record_log_probs");
+    if (shots == 0) return CLIFFT_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : library_stream();
+    LaunchShape L = p->base_shape;
+    void* gs = nullptr;
+    uint64_t max_teams = ~0ull;
+    if (L.global_amps) {
+        const size_t cap = kNoAllocScratchElems * sizeof(dv2);
+        const size_t one = (size_t)16 << p->peak_rank;
+        if (L.amp_slices == 2 && 2 * one > cap) L.amp_slices = 1;  // fused runs need a second slice
+        if (one * L.amp_slices > cap)
+            return fail(CLIFFT_E_UNSUPPORTED, "sample_noalloc: a peak_rank " + std::to_string(p->peak_rank) +
+                                                  " state needs " + std::to_string(one) +
+                                                  " bytes; its scratch holds " + std::to_string(cap) +
+                                                  " (peak_rank <= 22)");
+        if (L.amp_slices == 1 && p->base_shape.amp_slices == 2) L.lds_bytes -= kMonoBytes;
+        max_teams = cap / (one * L.amp_slices);
+        int rc = noalloc_scratch(&gs);
+        if (rc) return rc;
+    }
+    int rc = choose_global_team(L, shots);
+    if (rc) return rc;
+    const uint32_t tpb = L.teams_per_block();
+    int per_cu = 0;
+    HIP_TRY(shot_occupancy(L, &per_cu));
+    const uint64_t blocks_needed = (shots + tpb - 1) / tpb;
+    uint64_t grid = std::min<uint64_t>(blocks_needed, (uint64_t)device_info().num_cus * (uint64_t)std::max(per_cu, 1));
+    if (L.global_amps) grid = std::min<uint64_t>(grid, std::max<uint64_t>(1, max_teams / tpb));
+    L.grid = (uint32_t)std::max<uint64_t>(1, grid);
+    if (L.global_amps && (uint64_t)L.grid * tpb > max_teams)
+        return fail(CLIFFT_E_INVALID, "sample_noalloc: launch exceeds the scratch");
+    ShotArgs A;
+    fill_common_args(p, A, gs);
+    A.amp_slices = L.global_amps ? L.amp_slices : 1u;
+    A.shot_begin = 0;
+    A.shot_count = shots;
+    A.seed = seed;
+    A.mode = MODE_SAMPLE;
+    A.flags = flags;
+    if (out) {
+        A.out_meas = out->measurements;
+        A.out_det = out->detectors;
+        A.out_obs = out->observables;
+        A.out_ev = out->exp_vals;
+        A.out_disc = out->discarded;
+    }
+    HIP_TRY(launch_with_args(p, L, p->dp_dev, A, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return CLIFFT_OK;
+}
+
+}  // extern "C"
+
 static int survivors_impl(clifft_svm_program* p, uint64_t shots, uint64_t seed, uint32_t flags,
                           clifft_svm_survivors* summary, const clifft_svm_records* out, void* stream, uint32_t mode,
                           const KfTables* kf) {

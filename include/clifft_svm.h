@@ -309,6 +309,20 @@ int clifft_svm_program_info_get(const clifft_svm_program* prog, clifft_svm_progr
 int clifft_svm_sample(clifft_svm_program* prog, uint64_t shots, uint64_t seed, uint32_t flags,
                       const clifft_svm_records* out, void* stream);
 
+/* clifft_svm_sample that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Outputs must be device memory (flags must hold
+ * CLIFFT_REC_DEVICE); records equal clifft_svm_sample's.  Programs whose
+ * amplitudes live in global memory use a fixed 64 MiB device array of the
+ * library instead of the per-stream arena: peak_rank <= 22 (else
+ * CLIFFT_E_UNSUPPORTED), with as many teams in flight as the array holds.
+ * Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (on different streams or threads) race; the call returns after the
+ * records are written. */
+int clifft_svm_sample_noalloc(clifft_svm_program* prog, uint64_t shots, uint64_t seed, uint32_t flags,
+                              const clifft_svm_records* out, void* stream);
+
 /* Survivor-only sampling.  Replaces clifft::sample_survivors
  * (src/clifft/svm/svm.h:358, svm.cc:266-345).  When `out` is non-NULL the
  * surviving rows are compacted into its arrays in shot order and
