@@ -49,6 +49,10 @@ WORKLOADS = {
     "clifford_t20": ("20 qubits x 40 layers random Clifford+T, 6 T, final M (peak k=6)", 100_000),
     "large_k34": ("34 qubits x 40 layers, 26 T on distinct qubits, MR on 4 qubits / 10 layers", 256),
     "clifford_t24_k16": ("24 qubits x 40 layers random Clifford+T, 16 T, final M (peak k=16)", 2000),
+    "clifford_t24_k8": ("24 qubits x 40 layers random Clifford+T, 8 T on distinct qubits, final M (peak k=8)", 20000),
+    "clifford_t24_k10": ("24 qubits x 40 layers random Clifford+T, 10 T on distinct qubits, final M (peak k=10)",
+                         20000),
+    "clifford_t24_k13": ("24 qubits x 40 layers random Clifford+T, 13 T, final M (peak k=13)", 20000),
     "rz24_grad": ("24 qubits, 30 Clifford layers, 16 R_Z(theta_i) slots (peak k=16), 32 random Pauli probes "
                   "with nonzero expectation; "
                   "<O> and d<O>/dtheta for 512 theta vectors by the adjoint kernel", 512),
@@ -69,6 +73,10 @@ def build_program(name):
         return ccompile(W.large_k())
     if name == "clifford_t24_k16":
         return ccompile(W.random_clifford_t(n=24, layers=40, t_count=16, seed=42))
+    if name in ("clifford_t24_k8", "clifford_t24_k10", "clifford_t24_k13"):
+        t = int(name.rsplit("k", 1)[1])
+        # T on distinct qubits reaches the peak for 8 and 10 T; 13 T reach it as drawn
+        return ccompile(W.random_clifford_t(n=24, layers=40, t_count=t, seed=42, t_distinct_qubits=t != 13))
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -96,6 +104,11 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def _num(x):
+    """A finite float, or None (JSON has no NaN)."""
+    return float(x) if np.isfinite(x) else None
 
 
 def _timed(fn, reps):
@@ -232,10 +245,13 @@ def run_grad(args):
     print(json.dumps(line), flush=True)
 
 
-def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots):
+def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots, batch_only=False):
     """The batch (autotuned on a `tune_shots`-per-program sample) against one
     plain launch per program, on the same stream and outputs.  Returns
-    timings in seconds and the per-group kernel times of one eager run."""
+    timings in seconds and the per-group kernel times of one eager run with
+    the groups in sequence (side by side, a group's events would also count
+    the time it waits for the others).  batch_only: default tuning, no
+    per-program launches and no graph replays (a short run to profile)."""
     import ctypes
 
     import torch
@@ -254,7 +270,7 @@ def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots):
     sh = stream.cuda_stream
     bt = B.Batch(progs, [shots] * count, outs, packed=True)
     t_tune = time.perf_counter()
-    tuning = bt.tune(sample_shots=tune_shots, stream=sh)
+    tuning = bt.tuning if batch_only else bt.tune(sample_shots=tune_shots, stream=sh)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t_tune
     groups = bt.groups()
@@ -267,20 +283,25 @@ def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots):
         _lib.check(_lib.lib().clifft_svm_batch_sample(harr, shots_arr.ctypes.data, seeds.ctypes.data, count,
                                                       flags_pp, recs, sh), "per-program")
 
-    per_program(seeds0)
-    torch.cuda.synchronize()
-    reps_pp = max(1, min(3, steps))
-    t0 = time.perf_counter()
-    for r in range(reps_pp):
-        per_program(seeds0 + np.uint64(r + 1))
-    torch.cuda.synchronize()
-    t_pp = (time.perf_counter() - t0) / reps_pp
-    # per-group kernel times (timing events on each group's stream)
+    t_pp = float("nan")
+    if not batch_only:
+        per_program(seeds0)
+        torch.cuda.synchronize()
+        reps_pp = max(1, min(3, steps))
+        t0 = time.perf_counter()
+        for r in range(reps_pp):
+            per_program(seeds0 + np.uint64(r + 1))
+        torch.cuda.synchronize()
+        t_pp = (time.perf_counter() - t0) / reps_pp
+    # per-group kernel times: timing events around each group's launch, the
+    # groups in sequence for this one run
+    bt.set_tuning(concurrent=0)
     bt.time_groups(True)
     bt.run(seeds0 + np.uint64(99), stream=sh)
     torch.cuda.synchronize()
     group_ms = bt.group_times()
     bt.time_groups(False)
+    bt.set_tuning(**tuning)
     for i in range(warmup):
         bt.run(seeds0 + np.uint64(100 + i), stream=sh)
     torch.cuda.synchronize()
@@ -295,13 +316,15 @@ def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots):
             ends[i].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        bt.run_graph(seeds0, stream=sh)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(steps):
-            bt.run_graph(seeds0 + np.uint64(2000 + i), stream=sh)
-        torch.cuda.synchronize()
-        wall_graph = time.perf_counter() - t0
+        wall_graph = float("nan")
+        if not batch_only:
+            bt.run_graph(seeds0, stream=sh)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(steps):
+                bt.run_graph(seeds0 + np.uint64(2000 + i), stream=sh)
+            torch.cuda.synchronize()
+            wall_graph = time.perf_counter() - t0
     step_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     per_prog_bytes = [(h.info()["bytes_per_shot"] + nb(p.num_measurements)) * shots for h, p in zip(handles, progs)]
     bt.close()
@@ -414,7 +437,8 @@ def run_batch(args):
         shots = args.shots
     progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
     seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
-    r = _measure_batch(progs, shots, seeds0, args.steps, args.warmup, tune_shots=max(1, shots // 16))
+    r = _measure_batch(progs, shots, seeds0, args.steps, args.warmup, tune_shots=max(1, shots // 16),
+                       batch_only=args.batch_only)
     total = shots * count
     value = total / r["t_batch"]
     # dominant kernel: the launch group with the longest kernel time
@@ -441,10 +465,10 @@ def run_batch(args):
                    "mean_instructions": float(np.mean([len(p.bytecode) for p in progs])),
                    "tuning": r["tuning"], "tune_seconds": r["tune_s"], "launch_groups": r["groups"],
                    "group_kernel_ms": r["group_ms"], "records": "bit-packed measurements, device memory",
-                   "per_program_launches_shots_per_s": total / r["t_per_program"],
-                   "speedup_vs_per_program": r["t_per_program"] / r["t_batch"],
-                   "graph_replay_shots_per_s": total / r["t_graph"],
-                   "speedup_graph_vs_per_program": r["t_per_program"] / r["t_graph"],
+                   "per_program_launches_shots_per_s": _num(total / r["t_per_program"]),
+                   "speedup_vs_per_program": _num(r["t_per_program"] / r["t_batch"]),
+                   "graph_replay_shots_per_s": _num(total / r["t_graph"]),
+                   "speedup_graph_vs_per_program": _num(r["t_per_program"] / r["t_graph"]),
                    "parallelism": "programs x shots"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -467,6 +491,8 @@ def main():
     ap.add_argument("--workload", default="batch4096", choices=sorted(WORKLOADS))
     ap.add_argument("--shots", type=int, default=0, help="override shots per step")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch4096: default tuning, no per-program launches or graph replays (to profile)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()

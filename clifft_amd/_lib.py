@@ -57,7 +57,8 @@ class BatchTuning(ctypes.Structure):
     _fields_ = [("global_threads", ctypes.c_uint32), ("wave_rank_max", ctypes.c_uint32),
                 ("lds_rank_max", ctypes.c_uint32), ("wave_chunk", ctypes.c_uint32),
                 ("block_chunk", ctypes.c_uint32), ("concurrent", ctypes.c_uint32),
-                ("lane_rank_max", ctypes.c_int32), ("lane_chunk", ctypes.c_uint32)]
+                ("lane_rank_max", ctypes.c_int32), ("lane_chunk", ctypes.c_uint32),
+                ("gwave_rank_max", ctypes.c_uint32)]
 
 
 class BatchGroupInfo(ctypes.Structure):

@@ -103,12 +103,13 @@ class Batch:
             pass
 
     def groups(self) -> list:
-        """The current plan's launch groups (kind: wave / lds / global / lane)."""
+        """The current plan's launch groups (kind: wave / lds / global / lane /
+        gwave, the last one shot per wavefront on global amplitudes)."""
         n = ctypes.c_uint32()
         _lib.check(_lib.lib().clifft_svm_batch_info(self.handle, 0, None, ctypes.byref(n)), "batch_info")
         arr = (_lib.BatchGroupInfo * max(n.value, 1))()
         _lib.check(_lib.lib().clifft_svm_batch_info(self.handle, n.value, arr, ctypes.byref(n)), "batch_info")
-        kinds = ["wave", "lds", "global", "lane"]
+        kinds = ["wave", "lds", "global", "lane", "gwave"]
         out = []
         for g in arr[:n.value]:
             d = {f: getattr(g, f) for f, _ in g._fields_}

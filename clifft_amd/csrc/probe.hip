@@ -34,10 +34,20 @@ __global__ void __launch_bounds__(256) lds_probe_kernel(double* out, int iters) 
     if (acc.x == -1.0) out[blockIdx.x * 256 + t] = acc.y;  // keeps the loop alive
 }
 
+// Grid-stride copy with four independent 16-byte loads in flight per lane
+// (all four issued before the stores), lane-consecutive addresses.
 __global__ void __launch_bounds__(256) copy_probe_kernel(const dv2* __restrict__ src, dv2* __restrict__ dst,
                                                          uint64_t n) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * T < n; i += 4 * T) {
+        const dv2 a = src[i], b = src[i + T], c = src[i + 2 * T], d = src[i + 3 * T];
+        dst[i] = a;
+        dst[i + T] = b;
+        dst[i + 2 * T] = c;
+        dst[i + 3 * T] = d;
+    }
+    for (; i < n; i += T) dst[i] = src[i];
 }
 
 extern "C" {

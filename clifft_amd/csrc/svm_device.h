@@ -67,6 +67,7 @@ constexpr uint32_t kLaneDetectorRun = 0xF2;
 struct DevProgram {
     const clifft_instruction* bc;
     const clifft_instruction* lane_bc;  // lane interpreter copy, operands inlined
+    const clifft_instruction* team_bc;  // team interpreter copy: frame runs packed (one-word frames)
     const uint64_t* pauli_x;
     const uint64_t* pauli_z;
     const uint8_t* pauli_sign;

@@ -347,9 +347,12 @@ size_t rec_stride_of(const clifft_svm_program* p, bool kfault = false) {
 // Runs of frame opcodes become kLaneFrameRun and runs of inline-parity
 // DETECTORs kLaneDetectorRun (svm_device.h); the instructions they cover
 // stay in place.
-void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code) {
-    // runs of frame opcodes become one packed instruction (first of the run)
-    const uint64_t n = v->num_instructions;
+// Runs of 2..kLaneFrameRunMax frame opcodes on axes below 64 become one
+// kLaneFrameRun instruction holding the ops as 16-bit fields (opcode, axis
+// 1, axis 2) in its payload; it names the run's length in axis_1 and the
+// instructions it covers beyond itself in axis_2.  Used by the lane copy and
+// by the team interpreter's copy of one-word-frame programs.
+void pack_frame_runs(clifft_instruction* code, uint64_t n) {
     for (uint64_t i = 0; i < n;) {
         uint64_t j = i;
         while (j < n && j - i < kLaneFrameRunMax && code[j].opcode <= CLIFFT_OP_FRAME_SWAP && code[j].axis_1 < 64 &&
@@ -371,6 +374,11 @@ void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code
         std::memcpy(h.p.raw, w, 24);
         i = j;
     }
+}
+
+void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code) {
+    const uint64_t n = v->num_instructions;
+    pack_frame_runs(code, n);  // first, so the runs never hold the opcodes below
     for (uint64_t i = 0; i < v->num_instructions; ++i) {
         clifft_instruction& in = code[i];
         uint64_t w2 = 0, w3 = 0;
@@ -464,6 +472,16 @@ size_t global_workset_bytes() {
 
 // Fused permutation/phase runs in the global tier (CLIFFT_FUSE_MONO=0 turns
 // them off).
+// Packed frame runs in the team interpreter's bytecode (CLIFFT_TEAM_FRAME_RUNS=0
+// keeps one dispatch per frame opcode).
+bool team_frame_runs_enabled() {
+    static bool v = [] {
+        const char* e = std::getenv("CLIFFT_TEAM_FRAME_RUNS");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
 bool fuse_mono_enabled() {
     static bool v = [] {
         const char* e = std::getenv("CLIFFT_FUSE_MONO");
@@ -792,6 +810,8 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     size_t o_fbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
     size_t o_lbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
     size_t o_lfbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
+    size_t o_tbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
+    size_t o_tfbc = add(nullptr, view->num_instructions * sizeof(clifft_instruction));
     size_t o_px = add(view->pauli_x, view->num_pauli_masks * W * 8);
     size_t o_pz = add(view->pauli_z, view->num_pauli_masks * W * 8);
     size_t o_ps = add(view->pauli_sign, view->num_pauli_masks);
@@ -861,6 +881,15 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
                 op = (uint8_t)(op + (CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED - CLIFFT_OP_MEAS_DORMANT_STATIC));
         }
         if (view->words == 1) lane_fuse_meas_pauli(view, lbc);  // sampling copy only
+        // the team interpreter's copies: frame runs packed (one-word frames)
+        auto* tbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_tbc);
+        auto* tfbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_tfbc);
+        std::memcpy(tbc, view->bytecode, view->num_instructions * sizeof(clifft_instruction));
+        std::memcpy(tfbc, fbc, view->num_instructions * sizeof(clifft_instruction));
+        if (view->words == 1 && team_frame_runs_enabled()) {
+            pack_frame_runs(tbc, view->num_instructions);
+            pack_frame_runs(tfbc, view->num_instructions);
+        }
     }
     hipError_t e = hipMalloc(&p->dev_blob, total);
     if (e == hipSuccess) e = hipMemcpy(p->dev_blob, staging.data(), total, hipMemcpyHostToDevice);
@@ -877,6 +906,7 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     d.bc = reinterpret_cast<const clifft_instruction*>(base + o_bc);
     p->forced_bc = reinterpret_cast<clifft_instruction*>(base + o_fbc);
     d.lane_bc = reinterpret_cast<const clifft_instruction*>(base + o_lbc);
+    d.team_bc = reinterpret_cast<const clifft_instruction*>(base + o_tbc);
     d.pauli_x = reinterpret_cast<const uint64_t*>(base + o_px);
     d.pauli_z = reinterpret_cast<const uint64_t*>(base + o_pz);
     d.pauli_sign = base + o_ps;
@@ -915,6 +945,7 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         DevProgram both[2] = {d, d};
         both[1].bc = p->forced_bc;
         both[1].lane_bc = reinterpret_cast<const clifft_instruction*>(base + o_lfbc);
+        both[1].team_bc = reinterpret_cast<const clifft_instruction*>(base + o_tfbc);
         e = hipMemcpy(base + o_dp, both, sizeof(both), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             std::string msg = std::string("program upload: ") + hipGetErrorString(e);
@@ -1501,6 +1532,10 @@ clifft_svm_batch_tuning default_tuning() {
     t.concurrent = 1;
     t.lane_rank_max = lane_max_rank() < 0 ? -1 : std::min(lane_max_rank(), 6);
     t.lane_chunk = 256;
+    // Measured on one MI355X (4096-program workload, 1000 shots each): the
+    // tuner's pick, programs of peak 11-14 on global wavefront teams, took
+    // the batch from 2.16e6 to 2.96e6 shots/s.
+    t.gwave_rank_max = 14;
     return t;
 }
 
@@ -1513,6 +1548,7 @@ int batch_check_tuning(const clifft_svm_batch_tuning& t) {
         return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
     if (t.lane_rank_max > 6) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 6");
     if (t.concurrent > 1) return fail(CLIFFT_E_INVALID, "batch tuning: concurrent must be 0 or 1");
+    if (t.gwave_rank_max > kMaxPeakRank) return fail(CLIFFT_E_INVALID, "batch tuning: gwave_rank_max > 30");
     return CLIFFT_OK;
 }
 
@@ -1541,8 +1577,12 @@ bool batch_shape(const clifft_svm_program* p, const clifft_svm_batch_tuning& t, 
             return true;
         }
     }
-    L.block_team = true;
     L.global_amps = true;
+    if (k <= t.gwave_rank_max && red + 4 * (rec + kMonoBytes) <= kBatchMaxDynLds) {
+        L.block = 256;  // four wavefront teams per workgroup
+        return true;
+    }
+    L.block_team = true;
     L.block = t.global_threads;
     while (red + (L.block / 64) * rec > kBatchMaxDynLds && L.block > 256) L.block /= 2;
     return red + (L.block / 64) * rec <= kBatchMaxDynLds;
@@ -1563,7 +1603,7 @@ size_t group_lds(const BatchGroup& g) {
     if (L.lane) return (size_t)(L.block / 64) * g.wave_stride;
     size_t lds = kTeamScratchBytes + (size_t)(L.block / 64) * g.rec_stride;
     if (!L.global_amps) lds += (size_t)L.teams_per_block() * g.amp_elems * 16;
-    if (L.global_amps && g.amp_slices == 2) lds += kMonoBytes;
+    if (L.global_amps && g.amp_slices == 2) lds += (size_t)L.teams_per_block() * kMonoBytes;  // a list per team
     return lds;
 }
 
@@ -1707,6 +1747,7 @@ int batch_audit(const clifft_svm_batch* b) {
         if (L.lds_bytes > kBatchMaxDynLds) return fail(CLIFFT_E_INVALID, where + "LDS over the limit");
         if (L.block == 0 || L.block > 1024 || L.block % 64) return fail(CLIFFT_E_INVALID, where + "bad block");
         if (!L.block_team && !L.lane && L.block > 256) return fail(CLIFFT_E_INVALID, where + "bad wave-team block");
+        if (L.global_amps && !g.gscratch) return fail(CLIFFT_E_INVALID, where + "global group without scratch");
         if (L.global_amps && (size_t)g.teams_resident * g.amp_elems * g.amp_slices * 16 > g.gscratch_bytes)
             return fail(CLIFFT_E_INVALID, where + "global slices exceed the scratch");
         for (const BatchItem& it : g.items) {
@@ -1730,7 +1771,7 @@ int batch_audit(const clifft_svm_batch* b) {
                 return fail(CLIFFT_E_INVALID, where + "records exceed the record stride");
             size_t carve = kTeamScratchBytes + (size_t)(L.block / 64) * A.rec_stride;
             if (!L.global_amps) carve += (size_t)L.teams_per_block() * A.amp_elems * 16;
-            if (L.global_amps && A.amp_slices == 2) carve += kMonoBytes;
+            if (L.global_amps && A.amp_slices == 2) carve += (size_t)L.teams_per_block() * kMonoBytes;
             if (carve > L.lds_bytes) return fail(CLIFFT_E_INVALID, where + "LDS carve-up exceeds the launch's LDS");
             if (L.global_amps && A.amp_slices > g.amp_slices)
                 return fail(CLIFFT_E_INVALID, where + "program uses more slices than a team holds");
@@ -1760,8 +1801,9 @@ int batch_grids(clifft_svm_batch* b) {
             budget = budget > headroom ? budget - headroom : (size_t)1 << 30;
             budget = std::min<size_t>(budget, 192ull << 30);
             const size_t per_team = (size_t)g.amp_elems * g.amp_slices * 16;
-            grid = std::min<uint64_t>(grid, std::max<uint64_t>(1, budget / per_team));
-            const size_t need = (size_t)grid * per_team;
+            const uint64_t teams = g.L.teams_per_block();
+            grid = std::min<uint64_t>(grid, std::max<uint64_t>(1, budget / (per_team * teams)));
+            const size_t need = (size_t)grid * teams * per_team;
             if (need > g.gscratch_bytes) {
                 if (g.gscratch) {
                     HIP_TRY(hipDeviceSynchronize());
@@ -2055,7 +2097,7 @@ int clifft_svm_batch_info(const clifft_svm_batch* b, uint32_t max_groups, clifft
     for (uint32_t gi = 0; gi < max_groups && gi < b->groups.size() && info; ++gi) {
         const BatchGroup& g = b->groups[gi];
         clifft_svm_batch_group_info& o = info[gi];
-        o.kind = g.L.lane ? 3u : (g.L.global_amps ? 2u : (g.L.block_team ? 1u : 0u));
+        o.kind = g.L.lane ? 3u : (g.L.global_amps ? (g.L.block_team ? 2u : 4u) : (g.L.block_team ? 1u : 0u));
         o.programs = (uint32_t)g.progs.size();
         o.items = (uint32_t)g.items.size();
         o.grid = g.L.grid;
@@ -2205,6 +2247,7 @@ int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stre
     if (!rc) try_u32(&clifft_svm_batch_tuning::lane_chunk, {64, 128, 512, 1024});
     if (!rc) try_u32(&clifft_svm_batch_tuning::lds_rank_max, {8, 9, 11, 12, 13});
     if (!rc) try_u32(&clifft_svm_batch_tuning::wave_rank_max, {5, 6, 8});
+    if (!rc) try_u32(&clifft_svm_batch_tuning::gwave_rank_max, {12, 13, 14, 16});
     if (!rc) try_u32(&clifft_svm_batch_tuning::global_threads, {512, 1024});
     if (!rc) try_u32(&clifft_svm_batch_tuning::wave_chunk, {4, 8, 32, 64});
     if (!rc) try_u32(&clifft_svm_batch_tuning::block_chunk, {1, 2, 8, 16});

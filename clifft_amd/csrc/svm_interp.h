@@ -545,11 +545,16 @@ struct Interp {
     // ------------------------------------------------------------- dispatch
     // svm_kernels.inl:2183-2646.  Returns when the bytecode ends, a
     // POSTSELECT discards the shot, or a forced outcome is unreachable.
+    // One-word frames read the team copy of the bytecode, where runs of frame
+    // opcodes are packed into kLaneFrameRun instructions (one dispatch per
+    // run; the covered instructions stay in place and are skipped).
     CSVM_INLINE void run(cptr<DevProgram> Pg, double* ev_out) {
+        constexpr bool kPacked = std::is_same<F, FrameReg<1>>::value;
         cptr<DevProgram> P0 = opaque(Pg);
-        cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(P0->bc));
+        cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(kPacked ? P0->team_bc : P0->bc));
         const uint32_t n_instr = P0->n_instr;
-        for (uint32_t pc = 0; pc < n_instr; ++pc) {
+        uint32_t skip = 0;
+        for (uint32_t pc = 0; pc < n_instr; pc += 1 + skip) {
             const uint64_t w0 = code[4 * pc], w1 = code[4 * pc + 1];
             const uint32_t op = (uint32_t)(w0 & 0xFF);
             const uint32_t flags = (uint32_t)((w0 >> 16) & 0xFF);
@@ -557,6 +562,24 @@ struct Interp {
             const uint32_t a2 = (uint32_t)(w0 >> 48);
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
+            skip = 0;
+            if (kPacked && op == kLaneFrameRun) {
+                // frame opcodes keep deferred runs and pending expansions
+                const uint64_t w2 = code[4 * pc + 2], w3 = code[4 * pc + 3];
+                for (uint32_t j = 0; j < a1; ++j) {
+                    const uint64_t w = j < 4 ? w1 : (j < 8 ? w2 : w3);
+                    const uint32_t fr = (uint32_t)(w >> (16 * (j & 3u)));
+                    const uint32_t fop = fr & 7u, fa = (fr >> 3) & 63u, fb = (fr >> 9) & 63u;
+                    if (fop == CLIFFT_OP_FRAME_CNOT) frame_cnot(fa, fb);
+                    else if (fop == CLIFFT_OP_FRAME_CZ) frame_cz(fa, fb);
+                    else if (fop == CLIFFT_OP_FRAME_H) frame_h(fa);
+                    else if (fop == CLIFFT_OP_FRAME_S) frame_s(fa, 1.0);
+                    else if (fop == CLIFFT_OP_FRAME_S_DAG) frame_s(fa, -1.0);
+                    else frame_swap(fa, fb);
+                }
+                skip = a2;
+                continue;
+            }
             // an active measurement takes a run of two or more itself (probs)
             if (nmono && !((kMonoKeepOps >> op) & 1ull) && !(nmono >= 2 && ((kMonoMeasOps >> op) & 1ull)))
                 flush_mono();

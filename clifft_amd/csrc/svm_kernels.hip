@@ -82,14 +82,15 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
         S.amp = (AmpT*)(gamp);
     else
         S.amp = (AmpT*)((dv2*)lds_amp + (size_t)team_in_block * amp_elems);
-    // fused permutation/phase runs: workgroup teams on global amplitudes with
-    // a second slice; the opcode list follows the wavefronts' record blocks
+    // fused permutation/phase runs: teams on global amplitudes with a second
+    // slice; the opcode list (one per team) follows the wavefronts' record
+    // blocks
     S.amp_alt = nullptr;
     S.mops = nullptr;
     S.nmono = 0;
-    if (GA && BT && A->amp_slices == 2) {
+    if (GA && A->amp_slices == 2) {  // wavefront teams: one list per wavefront
         S.amp_alt = (AmpT*)(gamp + amp_elems);
-        S.mops = (lds_mono*)(p + (size_t)waves_block * rec_stride);
+        S.mops = (lds_mono*)(p + (size_t)waves_block * rec_stride + (BT ? 0u : wave_in_block * kMonoBytes));
     }
 
     for (uint64_t s = s_first; s < s_end; s += s_step) {
@@ -341,7 +342,7 @@ __global__ void __launch_bounds__(256, MINB) lane_kernel(const DevProgram* P, co
 
 // One program: teams stride over the launch's shots.
 template <int FK, bool BT, bool GA>
-__global__ void __launch_bounds__(BT ? 1024 : 256) shot_kernel(const DevProgram* P, const ShotArgs* A) {
+__global__ void __launch_bounds__(BT ? 1024 : 256, 4) shot_kernel(const DevProgram* P, const ShotArgs* A) {
     cptr<ShotArgs> Ag = cp(A);
     const uint32_t tpb = BT ? 1u : (blockDim.x >> 6);
     const uint32_t team_in_block = BT ? 0u : uni(threadIdx.x >> 6);
@@ -384,7 +385,7 @@ __global__ void __launch_bounds__(256, MINB) batch_lane_kernel(const BatchArgs* 
 }
 
 template <int FK, bool BT, bool GA>
-__global__ void __launch_bounds__(BT ? 1024 : 256) batch_team_kernel(const BatchArgs* B) {
+__global__ void __launch_bounds__(BT ? 1024 : 256, 4) batch_team_kernel(const BatchArgs* B) {
     __shared__ uint32_t head;
     cptr<BatchArgs> Bg = cp(B);
     const uint32_t tpb = BT ? 1u : (blockDim.x >> 6);

@@ -371,9 +371,10 @@ int clifft_svm_execute(clifft_svm_program* prog, clifft_svm_state* state);
  * the programs that share a kernel shape -- one shot per lane (peak_rank <=
  * lane_rank_max, at most 64 qubits; split by frame-word width), one shot
  * per wavefront (peak_rank <= wave_rank_max), one shot per workgroup with
- * LDS-resident amplitudes (<= lds_rank_max; split by team size), or one
- * shot per workgroup with amplitudes in a global slice (global_threads per
- * team).  A group's work items (a shot range of one program) are sorted by
+ * LDS-resident amplitudes (<= lds_rank_max; split by team size), one
+ * shot per wavefront with amplitudes in a global slice (<= gwave_rank_max),
+ * or one shot per workgroup with amplitudes in a global slice
+ * (global_threads per team).  A group's work items (a shot range of one program) are sorted by
  * descending estimated cost and drained by its teams from a device queue.
  * The groups run one after another on the caller's stream, or side by side
  * on the batch's own streams (concurrent = 1).
@@ -396,11 +397,14 @@ typedef struct {
     int32_t lane_rank_max;   /* <= 64-qubit programs up to this rank run one shot
                                 per lane (-1 = never), <= 6 */
     uint32_t lane_chunk;     /* shots per work item, lane tier (a multiple of 64) */
+    uint32_t gwave_rank_max; /* global-amplitude programs up to this rank run one shot
+                                per wavefront (<= 30; at or below lds_rank_max: never) */
 } clifft_svm_batch_tuning;
 
 /* One launch group of a batch's current plan. */
 typedef struct {
-    uint32_t kind;       /* 0 wavefront teams, 1 LDS workgroup teams, 2 global workgroup teams, 3 lanes */
+    uint32_t kind;       /* 0 wavefront teams, 1 LDS workgroup teams, 2 global workgroup teams, 3 lanes,
+                            4 global wavefront teams */
     uint32_t programs;   /* member programs */
     uint32_t items;      /* work items in its queue */
     uint32_t grid;       /* persistent workgroups */

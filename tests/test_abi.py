@@ -45,7 +45,7 @@ def test_compile_library_exports_every_declared_symbol():
 def test_abi_version_and_layouts():
     from clifft_amd.program import ABI_VERSION
     assert L.lib().clifft_svm_abi_version() == ABI_VERSION == 2
-    assert ctypes.sizeof(L.BatchTuning) == 32 and ctypes.sizeof(L.BatchGroupInfo) == 40
+    assert ctypes.sizeof(L.BatchTuning) == 36 and ctypes.sizeof(L.BatchGroupInfo) == 40
     # reference layouts: Instruction 32 B (backend.h:163), FusedU2Node 328 B, FusedU4Node 4480 B
     assert INSTR_DTYPE.itemsize == 32 and U2_DTYPE.itemsize == 328 and U4_DTYPE.itemsize == 4480
     offs = {n: INSTR_DTYPE.fields[n][1] for n in INSTR_DTYPE.names}

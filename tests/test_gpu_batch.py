@@ -113,7 +113,8 @@ TUNINGS = [dict(concurrent=0), dict(global_threads=512), dict(global_threads=102
            dict(wave_chunk=1, block_chunk=1), dict(wave_chunk=128, block_chunk=64),
            dict(global_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32),
            dict(lane_rank_max=-1), dict(lane_rank_max=5, lane_chunk=100), dict(lane_rank_max=1, lane_chunk=64),
-           dict(lane_rank_max=6, lane_chunk=1024), dict(concurrent=0, lane_rank_max=4, lane_chunk=3000)]
+           dict(lane_rank_max=6, lane_chunk=1024), dict(concurrent=0, lane_rank_max=4, lane_chunk=3000),
+           dict(gwave_rank_max=13), dict(gwave_rank_max=16, wave_chunk=3), dict(gwave_rank_max=30, lds_rank_max=8)]
 
 
 def test_batch_tunings_agree(mods):
@@ -197,17 +198,20 @@ def test_batch_graph_replays(mods, concurrent):
 def test_batch_groups_cover_every_shape(mods):
     """The mixed batch plans one launch group per kernel shape: lanes (by
     frame width), wavefront teams, LDS workgroup teams by size, global
-    teams; every program sits in exactly one group."""
+    wavefront teams (peak <= gwave_rank_max, 14 by default) and global
+    workgroup teams; every program sits in exactly one group."""
     svm, batch = mods
     progs, shots, seeds = mixed_programs(61, 26, wide=True)
     b, bufs = _device_batch(batch, progs, shots)
-    groups = b.groups()
-    assert {g["kind"] for g in groups} == {"lane", "wave", "lds", "global"}
-    assert sum(g["programs"] for g in groups) == len(progs)
-    for g in groups:
-        assert 0 < g["lds_bytes"] <= 160 * 1024 - 256 and g["grid"] > 0 and g["items"] > 0
-        if g["kind"] == "global":
-            assert g["block"] in (256, 512, 1024) and g["amp_slices"] in (1, 2)
+    for gwave, kinds in ((14, {"lane", "wave", "lds", "gwave", "global"}), (0, {"lane", "wave", "lds", "global"})):
+        b.set_tuning(gwave_rank_max=gwave)
+        groups = b.groups()
+        assert {g["kind"] for g in groups} == kinds, gwave
+        assert sum(g["programs"] for g in groups) == len(progs)
+        for g in groups:
+            assert 0 < g["lds_bytes"] <= 160 * 1024 - 256 and g["grid"] > 0 and g["items"] > 0
+            if g["kind"] == "global":
+                assert g["block"] in (256, 512, 1024) and g["amp_slices"] in (1, 2)
     b.close()
 
 
@@ -223,10 +227,21 @@ def test_batch_c3_programs_match_oracle(mods, orc):
     progs = [ccompile(text) for text, _ in W.batch_programs(count=64, seed=42, shots=24)]
     assert min(p.peak_rank for p in progs) <= 6 and max(p.peak_rank for p in progs) >= 14
     seeds = [42 ^ i for i in range(len(progs))]
+    ref = [orc.sample(p, 24, seed=seeds[i], per_shot=True)[0] for i, p in enumerate(progs)]
     got = batch.batch_sample(progs, 24, seeds, packed=True)
     for i, p in enumerate(progs):
-        m, d, o, e, _ = orc.sample(p, 24, seed=seeds[i], per_shot=True)
-        assert np.array_equal(got[i].measurements, np.packbits(m, axis=1, bitorder="little")), (i, p.peak_rank)
+        assert np.array_equal(got[i].measurements, np.packbits(ref[i], axis=1, bitorder="little")), (i, p.peak_rank)
+    # the same batch with the global-amplitude programs on wavefront teams
+    import torch
+    b, bufs = _device_batch(batch, progs, [24] * len(progs), packed=True)
+    b.set_tuning(gwave_rank_max=16)
+    assert "gwave" in {g["kind"] for g in b.groups()}
+    b.run(seeds)
+    torch.cuda.synchronize()
+    snap = _snapshot(bufs)
+    for i, p in enumerate(progs):
+        assert np.array_equal(snap[i][0], np.packbits(ref[i], axis=1, bitorder="little")), (i, p.peak_rank)
+    b.close()
 
 
 def test_batch_rejects_bad_tuning(mods):
