@@ -157,7 +157,7 @@ struct ShotArgs {
     double* preset_ev;                // MODE_PRESET exp_vals
     unsigned long long* dust_counter; // optional telemetry
     const KfTables* kf;               // MODE_KFAULT
-    uint32_t amp_slices;              // global tier: amplitude slices per team (2 = fused monomial runs)
+    uint32_t amp_slices;              // global tier: amplitude slices per team (2: fused runs; 3: and candidate folds)
     uint32_t pad0;
 };
 

@@ -300,6 +300,7 @@ struct clifft_svm_program {
     bool has_postselection = false;
     bool has_forced_ops = false;  // bytecode already contains *_FORCED opcodes
     bool has_mono_runs = false;   // some run of >= 2 permutation/phase opcodes could be fused
+    bool has_mono_meas = false;   // such a run directly precedes an active measurement
     bool frame32 = false;         // every frame axis and frame mask fits 32 bits
     uint64_t num_array_ops = 0;
     double bytes_per_shot = 0.0;
@@ -472,6 +473,22 @@ size_t global_workset_bytes() {
 
 // Fused permutation/phase runs in the global tier (CLIFFT_FUSE_MONO=0 turns
 // them off).
+// Candidate folds: a third slice for programs whose fused runs precede
+// active measurements (CLIFFT_FUSE_MEAS_FOLD=0 keeps two slices: the run's
+// pass, then the measurement's fold in place).
+bool fuse_meas_fold_enabled() {
+    static bool v = [] {
+        const char* e = std::getenv("CLIFFT_FUSE_MEAS_FOLD");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
+// Global-amplitude slices per team for a program with fusable runs.
+uint32_t mono_slices(const clifft_svm_program* p) {
+    return p->has_mono_meas && p->peak_rank <= 23 && fuse_meas_fold_enabled() ? 3u : 2u;
+}
+
 // Packed frame runs in the team interpreter's bytecode (CLIFFT_TEAM_FRAME_RUNS=0
 // keeps one dispatch per frame opcode).
 bool team_frame_runs_enabled() {
@@ -577,9 +594,12 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
     // state per run instead of one pass per opcode).
     // Only programs with a run of two or more, and states of at most 2^24
     // amplitudes (a second 512 MiB slice per team at most).
+    // A run that precedes an active measurement writes both folded states
+    // of the measurement in its pass, into a second and a third slice (states
+    // of at most 2^23 amplitudes).
     if (L.global_amps && L.block_team && fuse_mono_enabled() && p->has_mono_runs && k <= 24 &&
         lds + kMonoBytes <= kLdsBytes) {
-        L.amp_slices = 2;
+        L.amp_slices = mono_slices(p);
         lds += kMonoBytes;
     }
     L.lds_bytes = lds;
@@ -631,7 +651,7 @@ int choose_global_team(LaunchShape& L, uint64_t shots) {
     for (uint32_t t : {256u, 512u, 1024u}) {
         LaunchShape c = L;
         c.block = t;
-        c.lds_bytes = kTeamScratchBytes + (t / 64) * (size_t)L.rec_stride + (L.amp_slices == 2 ? kMonoBytes : 0);
+        c.lds_bytes = kTeamScratchBytes + (t / 64) * (size_t)L.rec_stride + (L.amp_slices >= 2 ? kMonoBytes : 0);
         if (c.lds_bytes > kLdsBytes) break;  // larger teams hold more record copies
         int per_cu = 0;
         hipError_t e = shot_occupancy(c, &per_cu);
@@ -787,6 +807,7 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         if ((kMonoOps & bit) && !dormant_phase) {
             if (++run >= 2) p->has_mono_runs = true;
         } else if (!(kMonoKeepOps & bit)) {
+            if (run >= 2 && (kMonoMeasOps & bit)) p->has_mono_meas = true;
             run = 0;
         }
     }
@@ -1133,13 +1154,13 @@ int clifft_svm_sample_noalloc(clifft_svm_program* p, uint64_t shots, uint64_t se
     if (L.global_amps) {
         const size_t cap = kNoAllocScratchElems * sizeof(dv2);
         const size_t one = (size_t)16 << p->peak_rank;
-        if (L.amp_slices == 2 && 2 * one > cap) L.amp_slices = 1;  // fused runs need a second slice
+        while (L.amp_slices > 1 && L.amp_slices * one > cap) --L.amp_slices;  // fused runs: 2 or 3 slices
         if (one * L.amp_slices > cap)
             return fail(CLIFFT_E_UNSUPPORTED, "sample_noalloc: a peak_rank " + std::to_string(p->peak_rank) +
                                                   " state needs " + std::to_string(one) +
                                                   " bytes; its scratch holds " + std::to_string(cap) +
                                                   " (peak_rank <= 22)");
-        if (L.amp_slices == 1 && p->base_shape.amp_slices == 2) L.lds_bytes -= kMonoBytes;
+        if (L.amp_slices == 1 && p->base_shape.amp_slices >= 2) L.lds_bytes -= kMonoBytes;
         max_teams = cap / (one * L.amp_slices);
         int rc = noalloc_scratch(&gs);
         if (rc) return rc;
@@ -1471,7 +1492,7 @@ struct BatchGroup {
     uint32_t amp_elems = 1;      // team tiers: largest 2^peak_rank of the members
     uint32_t rec_stride = 0;     // team tiers: largest record stride
     uint32_t wave_stride = 0;    // lane tier: largest lane_wave_bytes
-    uint32_t amp_slices = 1;     // global tier: 2 when a member runs fused permutation/phase runs
+    uint32_t amp_slices = 1;     // global tier: 2 or 3 when a member runs fused permutation/phase runs
     uint32_t teams_resident = 0; // teams in flight (grid x teams per workgroup)
     std::vector<BatchItem> items;
     size_t items_off = 0;        // offset of this group's items in the plan allocation
@@ -1603,7 +1624,7 @@ size_t group_lds(const BatchGroup& g) {
     if (L.lane) return (size_t)(L.block / 64) * g.wave_stride;
     size_t lds = kTeamScratchBytes + (size_t)(L.block / 64) * g.rec_stride;
     if (!L.global_amps) lds += (size_t)L.teams_per_block() * g.amp_elems * 16;
-    if (L.global_amps && g.amp_slices == 2) lds += (size_t)L.teams_per_block() * kMonoBytes;  // a list per team
+    if (L.global_amps && g.amp_slices >= 2) lds += (size_t)L.teams_per_block() * kMonoBytes;  // a list per team
     return lds;
 }
 
@@ -1628,7 +1649,10 @@ void group_sizes(BatchGroup& g, const std::vector<clifft_svm_program*>& progs) {
     if (g.L.global_amps) {
         bool any = false;
         for (uint32_t i : g.progs) any |= mono_eligible(progs[i]);
-        g.amp_slices = any ? 2 : 1;
+        uint32_t slices = 1;
+        for (uint32_t i : g.progs)
+            if (mono_eligible(progs[i])) slices = std::max(slices, mono_slices(progs[i]));
+        g.amp_slices = slices;
         if (any && group_lds(g) > kBatchMaxDynLds) g.amp_slices = 1;
     }
     g.L.amp_slices = g.amp_slices;
@@ -1771,7 +1795,7 @@ int batch_audit(const clifft_svm_batch* b) {
                 return fail(CLIFFT_E_INVALID, where + "records exceed the record stride");
             size_t carve = kTeamScratchBytes + (size_t)(L.block / 64) * A.rec_stride;
             if (!L.global_amps) carve += (size_t)L.teams_per_block() * A.amp_elems * 16;
-            if (L.global_amps && A.amp_slices == 2) carve += (size_t)L.teams_per_block() * kMonoBytes;
+            if (L.global_amps && A.amp_slices >= 2) carve += (size_t)L.teams_per_block() * kMonoBytes;
             if (carve > L.lds_bytes) return fail(CLIFFT_E_INVALID, where + "LDS carve-up exceeds the launch's LDS");
             if (L.global_amps && A.amp_slices > g.amp_slices)
                 return fail(CLIFFT_E_INVALID, where + "program uses more slices than a team holds");
@@ -1869,7 +1893,8 @@ int batch_plan(clifft_svm_batch* b, uint64_t cap, hipStream_t st) {
         if (!g.L.lane) {
             A.amp_elems = g.amp_elems;
             A.rec_stride = g.rec_stride;
-            A.amp_slices = g.L.global_amps && g.amp_slices == 2 && mono_eligible(p) ? 2u : 1u;
+            A.amp_slices = g.L.global_amps && g.amp_slices >= 2 && mono_eligible(p)
+                               ? std::min(g.amp_slices, mono_slices(p)) : 1u;
         }
     }
     rc = batch_audit(b);

@@ -22,7 +22,7 @@ struct LaunchShape {
     bool lane = false;         // one shot per lane (small rank, <= 64 qubits)
     bool lane_dense = false;   // lane kernel built for 6 resident workgroups (small LDS columns)
     bool lane_narrow = false;  // lane kernel with 32-bit frame words (at most 32 qubits)
-    uint32_t amp_slices = 1;   // global-amplitude slices per team (2: fused permutation/phase runs)
+    uint32_t amp_slices = 1;   // global-amplitude slices per team (2: fused permutation/phase runs, 3: and candidate folds)
     uint32_t rec_stride = 0;   // LDS bytes of one wavefront's record block (team tiers)
     uint32_t block = 256;      // threads per workgroup
     uint32_t grid = 1;         // workgroups

@@ -116,6 +116,11 @@ struct Interp {
     AmpT* amp_alt;
     lds_mono* mops;
     uint32_t nmono;
+    // A third slice (null: none): the run before an active measurement then
+    // writes both folded states, amp_alt (outcome 0) and amp_alt2 (outcome
+    // 1), and `cands` is set until the measurement takes one of them.
+    AmpT* amp_alt2;
+    uint32_t cands;
 
     // ----------------------------------------------------------------- gamma
     CSVM_INLINE void mul_phase(double re, double im) {
@@ -183,6 +188,12 @@ struct Interp {
     CSVM_INLINE double2 probs(uint32_t kind, uint32_t fa) {
         if (nmono >= 2) {
             team_sync<BT>();  // the list written by lane 0 is visible to every wave
+            if (amp_alt2) {
+                const double2 pr = sweep_mono_cands<BT>(amp, amp_alt, amp_alt2, k, mops, nmono, kind, fa, g, red);
+                cands = 1;
+                nmono = 0;
+                return pr;
+            }
             const double2 pr = sweep_mono_probs<BT>(amp, amp_alt, k, mops, nmono, kind, fa, g, red);
             AmpT* t = amp;
             amp = amp_alt;
@@ -191,6 +202,17 @@ struct Interp {
             return pr;
         }
         return sweep_probs<BT>(amp, k, kind, fa, g, red);
+    }
+
+    // Keep the folded state of outcome b written by sweep_mono_cands; the
+    // run's source and the other candidate become the free slices.
+    CSVM_INLINE void take_cand(uint32_t b) {
+        AmpT* src = amp;
+        AmpT* other = b ? amp_alt : amp_alt2;
+        amp = b ? amp_alt2 : amp_alt;
+        amp_alt = src;
+        amp_alt2 = other;
+        cands = 0;
     }
 
     // svm.h:177-186
@@ -361,7 +383,8 @@ struct Interp {
         uint32_t b, m_abs, rec;
         if (!choose(pr.x, pr.y, total, px, sign, idx, is_forced, b, m_abs, rec)) return false;
         if (b == 1 && pz) negate_gamma();
-        if (b == 1) sweep_take_upper<BT>(amp, k, g);
+        if (cands) take_cand(b);
+        else if (b == 1) sweep_take_upper<BT>(amp, k, g);
         k--;
         double pb = b ? pr.y : pr.x;
         if (is_forced || pb > 0.0) scale_magnitude(sqrt(total / pb));
@@ -377,7 +400,8 @@ struct Interp {
         double total = pr.x + pr.y;
         uint32_t bx, m_abs, rec;
         if (!choose(pr.x, pr.y, total, pz, sign, idx, is_forced, bx, m_abs, rec)) return false;
-        sweep_fold<BT>(amp, k, bx ? -1.0 : 1.0, g);
+        if (cands) take_cand(bx);
+        else sweep_fold<BT>(amp, k, bx ? -1.0 : 1.0, g);
         if (px && m_abs) negate_gamma();
         k--;
         double pb = bx ? pr.y : pr.x;
@@ -396,7 +420,8 @@ struct Interp {
         double total = pr.x + pr.y;
         uint32_t bx, m_abs, rec;
         if (!choose(pr.x, pr.y, total, pz, sign, idx, is_forced, bx, m_abs, rec)) return false;
-        sweep_swap_fold<BT>(amp, k, fa, bx ? -1.0 : 1.0, g);
+        if (cands) take_cand(bx);
+        else sweep_swap_fold<BT>(amp, k, fa, bx ? -1.0 : 1.0, g);
         k--;
         double pb = bx ? pr.y : pr.x;
         if (is_forced || pb > 0.0) scale_magnitude(sqrt(total / pb));

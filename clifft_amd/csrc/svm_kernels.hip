@@ -86,10 +86,13 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
     // slice; the opcode list (one per team) follows the wavefronts' record
     // blocks
     S.amp_alt = nullptr;
+    S.amp_alt2 = nullptr;
     S.mops = nullptr;
     S.nmono = 0;
-    if (GA && A->amp_slices == 2) {  // wavefront teams: one list per wavefront
+    S.cands = 0;
+    if (GA && A->amp_slices >= 2) {  // wavefront teams: one list per wavefront
         S.amp_alt = (AmpT*)(gamp + amp_elems);
+        if (A->amp_slices == 3) S.amp_alt2 = (AmpT*)(gamp + 2 * (size_t)amp_elems);
         S.mops = (lds_mono*)(p + (size_t)waves_block * rec_stride + (BT ? 0u : wave_in_block * kMonoBytes));
     }
 
@@ -105,6 +108,7 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
         S.kf = 0;
         S.npend = 0;
         S.nmono = 0;  // a shot that ended early leaves no deferred run to the next one
+        S.cands = 0;
         S.pend_mask = 0;
         S.dust = 0;
         S.discarded = 0;

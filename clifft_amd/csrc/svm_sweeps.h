@@ -411,6 +411,87 @@ CSVM_SWEEP double2 sweep_mono_probs(const AmpT* src, AmpT* dst, uint32_t k, cons
     return make_double2(v[0], v[1]);
 }
 
+// A deferred run, the outcome probabilities of the active measurement that
+// follows it, and both of that measurement's folded states, in one pass.
+// For each pair the probability sum visits (output index i < 2^(k-1)),
+// c0[i] and c1[i] receive the entry that the measurement leaves at i for
+// outcome 0 and for outcome 1: the run's result entries p (lower) and q
+// (upper) for the diagonal kind (keep the lower half or take the upper);
+// (p + q) / sqrt 2 and (p - q) / sqrt 2 for the interfere kinds, with the
+// index mapping of sweep_swap_fold for SWAP_MEAS_INTERFERE.  These are the
+// products sweep_take_upper, sweep_fold and sweep_swap_fold form from the
+// stored run result, so the state kept is bit-identical to gathering,
+// measuring and folding in place; the sums are sweep_mono_probs'.  One read
+// and one write of 2^k entries replace the gather's read and write plus the
+// fold's read of 2^k and write of 2^(k-1).
+template <bool BT, class AmpT>
+CSVM_SWEEP double2 sweep_mono_cands(const AmpT* src, AmpT* c0, AmpT* c1, uint32_t k, const lds_mono* ops, uint32_t n,
+                                    uint32_t kind, uint32_t fa, TeamGeom g, lds_f64* red) {
+    k = uni(k); n = uni(n); kind = uni(kind); fa = uni(fa); g = uni_geom(g);
+    const uint32_t half = 1u << (k - 1), t = k - 1, fb = 1u << fa;
+    constexpr uint32_t U = Unroll<AmpT>::value > 1 ? Unroll<AmpT>::value / 2 : 1;
+    double v[2] = {0.0, 0.0};
+    for (uint32_t i0 = g.lane; i0 < half; i0 += U * g.size) {
+        uint32_t y[2 * U], x[2 * U];
+        dv2 a[2 * U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t i = (i0 + u * g.size) & (half - 1u);
+            if (kind == PROB_SWAP_INTERFERE) {
+                const uint32_t bf = (i >> fa) & 1u;
+                y[2 * u] = (i & ~fb) | (bf << t);
+                y[2 * u + 1] = y[2 * u] | fb;
+            } else {
+                y[2 * u] = i;
+                y[2 * u + 1] = i + half;
+            }
+            x[2 * u] = y[2 * u];
+            x[2 * u + 1] = y[2 * u + 1];
+        }
+        for (uint32_t j = n; j-- > 0;) {
+            const MonoOp o = load_mono(ops + j);
+#pragma unroll
+            for (uint32_t e = 0; e < 2 * U; ++e) x[e] = mono_perm(o, x[e]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * g.size < half) {
+                a[2 * u] = src[x[2 * u]];
+                a[2 * u + 1] = src[x[2 * u + 1]];
+            }
+        for (uint32_t j = 0; j < n; ++j) {
+            const MonoOp o = load_mono(ops + j);
+#pragma unroll
+            for (uint32_t e = 0; e < 2 * U; ++e) {
+                a[e] = mono_phase(o, x[e], a[e]);
+                x[e] = mono_perm(o, x[e]);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * g.size;
+            if (i < half) {
+                const dv2 p = a[2 * u], q = a[2 * u + 1];
+                if (kind == PROB_DIAG) {
+                    c0[i] = p;
+                    c1[i] = q;
+                    v[0] += p.x * p.x + p.y * p.y;
+                    v[1] += q.x * q.x + q.y * q.y;
+                } else {
+                    c0[i] = mk((p.x + 1.0 * q.x) * kInvSqrt2, (p.y + 1.0 * q.y) * kInvSqrt2);
+                    c1[i] = mk((p.x + -1.0 * q.x) * kInvSqrt2, (p.y + -1.0 * q.y) * kInvSqrt2);
+                    const dv2 s = cadd(p, q), d = csub(p, q);
+                    v[0] += s.x * s.x + s.y * s.y;
+                    v[1] += d.x * d.x + d.y * d.y;
+                }
+            }
+        }
+    }
+    team_sync<BT>();
+    team_reduce<BT, 2>(v, g, red);
+    return make_double2(v[0], v[1]);
+}
+
 // EXPAND: amp[i + 2^k] = amp[i] * ph for i < 2^k (:876-945)
 template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_expand(AmpT* amp, uint32_t k, double pr, double pi, uint32_t has_phase, TeamGeom g) {
