@@ -158,7 +158,7 @@ struct ShotArgs {
     unsigned long long* dust_counter; // optional telemetry
     const KfTables* kf;               // MODE_KFAULT
     uint32_t amp_slices;              // global tier: amplitude slices per team (2: fused runs; 3: and candidate folds)
-    uint32_t pad0;
+    uint32_t pend_defer;              // with amp_slices >= 2: measurement passes read through pending expansions
 };
 
 // Opcodes that never read or write amplitudes nor depend on them, so
@@ -171,6 +171,8 @@ constexpr uint64_t kNoAmpOps =
     (1ull << CLIFFT_OP_MEAS_DORMANT_RANDOM) | (1ull << CLIFFT_OP_APPLY_PAULI) | (1ull << CLIFFT_OP_NOISE) |
     (1ull << CLIFFT_OP_NOISE_BLOCK) | (1ull << CLIFFT_OP_READOUT_NOISE) | (1ull << CLIFFT_OP_DETECTOR) |
     (1ull << CLIFFT_OP_OBSERVABLE);
+constexpr uint64_t kExpandOps = (1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) |
+                                (1ull << CLIFFT_OP_EXPAND_T_DAG) | (1ull << CLIFFT_OP_EXPAND_ROT);
 // Permutation/phase opcodes that a fused run may hold, and what may sit
 // between them (the amplitude-free opcodes other than EXPAND*).
 constexpr uint64_t kMonoOps =
@@ -178,9 +180,7 @@ constexpr uint64_t kMonoOps =
     (1ull << CLIFFT_OP_ARRAY_MULTI_CNOT) | (1ull << CLIFFT_OP_ARRAY_MULTI_CZ) | (1ull << CLIFFT_OP_ARRAY_S) |
     (1ull << CLIFFT_OP_ARRAY_S_DAG) | (1ull << CLIFFT_OP_ARRAY_T) | (1ull << CLIFFT_OP_ARRAY_T_DAG) |
     (1ull << CLIFFT_OP_ARRAY_ROT);
-constexpr uint64_t kMonoKeepOps =
-    kMonoOps | (kNoAmpOps & ~((1ull << CLIFFT_OP_EXPAND) | (1ull << CLIFFT_OP_EXPAND_T) |
-                              (1ull << CLIFFT_OP_EXPAND_T_DAG) | (1ull << CLIFFT_OP_EXPAND_ROT)));
+constexpr uint64_t kMonoKeepOps = kMonoOps | (kNoAmpOps & ~kExpandOps);
 // Active measurements, which apply a deferred run in their probability pass.
 constexpr uint64_t kMonoMeasOps =
     (1ull << CLIFFT_OP_MEAS_ACTIVE_DIAGONAL) | (1ull << CLIFFT_OP_MEAS_ACTIVE_INTERFERE) |

@@ -301,6 +301,7 @@ struct clifft_svm_program {
     bool has_forced_ops = false;  // bytecode already contains *_FORCED opcodes
     bool has_mono_runs = false;   // some run of >= 2 permutation/phase opcodes could be fused
     bool has_mono_meas = false;   // such a run directly precedes an active measurement
+    bool has_exp_meas = false;    // an EXPAND reaches an active measurement across such runs
     bool frame32 = false;         // every frame axis and frame mask fits 32 bits
     uint64_t num_array_ops = 0;
     double bytes_per_shot = 0.0;
@@ -484,9 +485,26 @@ bool fuse_meas_fold_enabled() {
     return v;
 }
 
+// Pending expansions read through by the next active measurement's pass
+// (CLIFFT_DEFER_EXPAND=0: every expansion run is written by its own pass).
+bool defer_expand_enabled() {
+    static bool v = [] {
+        const char* e = std::getenv("CLIFFT_DEFER_EXPAND");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
+// Whether a program gains from a second global slice: a fusable run, or an
+// expansion read through by a measurement pass.
+bool has_fusable(const clifft_svm_program* p) {
+    return p->has_mono_runs || (p->has_exp_meas && defer_expand_enabled());
+}
+
 // Global-amplitude slices per team for a program with fusable runs.
 uint32_t mono_slices(const clifft_svm_program* p) {
-    return p->has_mono_meas && p->peak_rank <= 23 && fuse_meas_fold_enabled() ? 3u : 2u;
+    const bool meas = p->has_mono_meas || (p->has_exp_meas && defer_expand_enabled());
+    return meas && p->peak_rank <= 23 && fuse_meas_fold_enabled() ? 3u : 2u;
 }
 
 // Packed frame runs in the team interpreter's bytecode (CLIFFT_TEAM_FRAME_RUNS=0
@@ -597,7 +615,7 @@ LaunchShape plan_shape(const clifft_svm_program* p, bool allow_lane = true, bool
     // A run that precedes an active measurement writes both folded states
     // of the measurement in its pass, into a second and a third slice (states
     // of at most 2^23 amplitudes).
-    if (L.global_amps && L.block_team && fuse_mono_enabled() && p->has_mono_runs && k <= 24 &&
+    if (L.global_amps && L.block_team && fuse_mono_enabled() && has_fusable(p) && k <= 24 &&
         lds + kMonoBytes <= kLdsBytes) {
         L.amp_slices = mono_slices(p);
         lds += kMonoBytes;
@@ -716,6 +734,7 @@ void fill_common_args(clifft_svm_program* p, ShotArgs& A, void* gscratch, bool k
     A.gscratch = reinterpret_cast<dv2*>(gscratch);
     A.dust_counter = p->dust_dev;
     A.amp_slices = 1;
+    A.pend_defer = defer_expand_enabled() ? 1u : 0u;
 }
 
 size_t row_bytes(uint32_t n, bool packed) { return packed ? (n + 7) / 8 : n; }
@@ -793,6 +812,7 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
     }
     const uint32_t fbits = view->words * 64;
     uint32_t run = 0;
+    bool expand_pending = false;
     for (uint64_t i = 0; i < view->num_instructions; ++i) {
         const clifft_instruction& in = p->host_bc[i];
         double b = op_bytes(in, khist[i], fbits);
@@ -809,6 +829,16 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         } else if (!(kMonoKeepOps & bit)) {
             if (run >= 2 && (kMonoMeasOps & bit)) p->has_mono_meas = true;
             run = 0;
+        }
+        // the interpreter's rule for pending expansions (with a second slice)
+        if (in.opcode == CLIFFT_OP_EXPAND || in.opcode == CLIFFT_OP_EXPAND_T || in.opcode == CLIFFT_OP_EXPAND_T_DAG ||
+            in.opcode == CLIFFT_OP_EXPAND_ROT) {
+            expand_pending = true;
+        } else if (kMonoMeasOps & bit) {
+            if (expand_pending) p->has_exp_meas = true;
+            expand_pending = false;
+        } else if (!((kNoAmpOps | kMonoOps) & bit)) {
+            expand_pending = false;
         }
     }
 
@@ -1615,7 +1645,7 @@ bool same_kernel(const LaunchShape& a, const LaunchShape& b) {
 }
 
 bool mono_eligible(const clifft_svm_program* p) {
-    return fuse_mono_enabled() && p->has_mono_runs && p->peak_rank <= 24;
+    return fuse_mono_enabled() && has_fusable(p) && p->peak_rank <= 24;
 }
 
 // Dynamic LDS of a group's launch from its largest members.

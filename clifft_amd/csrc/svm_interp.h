@@ -87,12 +87,18 @@ CSVM_INLINE void wave_fence() {
 
 template <class F, bool BT, class AmpT>
 struct Interp {
+    // amplitudes in global memory: the only teams with deferred runs
+    static constexpr bool kGlobalAmps = std::is_same<AmpT, glb_d2>::value;
     TeamGeom g;
     AmpT* amp;
     lds_f64* red;
     lds_f64* pend;       // this wavefront's pending-expansion phases
     uint32_t npend;      // expansions recorded but not yet written
     uint32_t pend_mask;  // which of them carry a phase
+    // 1: pending expansions stay pending under deferred permutation/phase
+    // runs and are read through by the next active measurement's pass
+    // (teams with a second slice only)
+    uint32_t pend_defer;
     __attribute__((address_space(3))) uint8_t* meas;
     __attribute__((address_space(3))) uint8_t* det;
     __attribute__((address_space(3))) uint8_t* obs;
@@ -132,20 +138,34 @@ struct Interp {
         g_re = -g_re;
         g_im = -g_im;
     }
-    // Write the pending expansions (ranks k - npend .. k).
-    CSVM_INLINE void flush_expand() {
-        if (npend == 0) return;
-        sweep_expand_multi<BT>(amp, k - npend, npend, pend, pend_mask, g);
-        npend = 0;
-        pend_mask = 0;
+    // Write the oldest n pending expansions, kMaxFusedExpand per pass; the
+    // rest stay pending (their phases move to the front of the list).
+    CSVM_INLINE void flush_expand_oldest(uint32_t n) {
+        if constexpr (!kGlobalAmps) {  // LDS amplitudes: n is always all of them (at most kMaxFusedExpand)
+            sweep_expand_multi<BT>(amp, k - npend, npend, pend, pend_mask, g);
+            npend = 0;
+            pend_mask = 0;
+            return;
+        }
+        while (n > 0) {
+            const uint32_t m = n < kMaxFusedExpand ? n : kMaxFusedExpand;
+            sweep_expand_multi<BT>(amp, k - npend, m, pend, pend_mask & ((1u << m) - 1u), g);
+            npend -= m;
+            pend_mask >>= m;
+            n -= m;
+            if ((threadIdx.x & 63u) == 0)
+                for (uint32_t j = 0; j < 2 * npend; ++j) pend[j] = pend[j + 2 * m];
+            wave_fence();
+        }
     }
-
-
     // Record one permutation/phase opcode of the current rank k instead of
     // sweeping now (false: fusion is off for this team; sweep as usual).
     CSVM_INLINE bool defer_mono(uint32_t kind, uint32_t a, uint32_t b, uint32_t m, double pr, double pi) {
-        if (!amp_alt) return false;
-        if (nmono == kMaxMono) flush_mono();
+        if (!kGlobalAmps || !amp_alt) return false;
+        if (nmono == kMaxMono) {
+            if (npend) flush_expand_oldest(npend);
+            flush_mono();
+        }
         if (g.lane == 0) {
             MonoOp o;
             o.kind = kind;
@@ -160,9 +180,10 @@ struct Interp {
         return true;
     }
     // Apply the deferred run: a single opcode in place with its own sweep,
-    // two or more in one out-of-place pass into the other slice.
+    // two or more in one out-of-place pass into the other slice.  The run
+    // was recorded over the expanded state: no expansion may be pending.
     CSVM_INLINE void flush_mono() {
-        if (nmono == 0) return;
+        if (!kGlobalAmps || nmono == 0) return;
         team_sync<BT>();  // the list written by lane 0 is visible to every wave
         if (nmono == 1) {
             const MonoOp o = load_mono(mops);
@@ -184,22 +205,39 @@ struct Interp {
 
     // Outcome probabilities of an active measurement.  A deferred run of two
     // or more permutation/phase opcodes is applied in the same pass (into the
-    // other slice) instead of a gather of its own followed by a reading pass.
-    CSVM_INLINE double2 probs(uint32_t kind, uint32_t fa) {
-        if (nmono >= 2) {
-            team_sync<BT>();  // the list written by lane 0 is visible to every wave
-            if (amp_alt2) {
-                const double2 pr = sweep_mono_cands<BT>(amp, amp_alt, amp_alt2, k, mops, nmono, kind, fa, g, red);
-                cands = 1;
-                nmono = 0;
-                return pr;
-            }
-            const double2 pr = sweep_mono_probs<BT>(amp, amp_alt, k, mops, nmono, kind, fa, g, red);
+    // other slice) instead of a gather of its own followed by a reading pass;
+    // pending expansions under it (or under no run) are read through, so the
+    // pass reads the state from before them.
+    template <class View>
+    CSVM_INLINE double2 fused_probs(const View& src, uint32_t kind, uint32_t fa) {
+        double2 pr;
+        if (amp_alt2) {
+            pr = sweep_mono_cands<BT>(src, amp_alt, amp_alt2, k, mops, nmono, kind, fa, g, red);
+            cands = 1;
+        } else {
+            pr = sweep_mono_probs<BT>(src, amp_alt, k, mops, nmono, kind, fa, g, red);
             AmpT* t = amp;
             amp = amp_alt;
             amp_alt = t;
-            nmono = 0;
-            return pr;
+        }
+        nmono = 0;
+        npend = 0;
+        pend_mask = 0;
+        return pr;
+    }
+    CSVM_INLINE double2 probs(uint32_t kind, uint32_t fa) {
+        // (teams with LDS amplitudes never defer: not compiled into them)
+        if constexpr (kGlobalAmps) {
+            if (npend) {
+                team_sync<BT>();  // the list written by lane 0 is visible to every wave
+                return fused_probs(ExpandView<AmpT>(amp, k - npend, npend, pend, pend_mask), kind, fa);
+            }
+            if (nmono >= 2) {
+                team_sync<BT>();
+                StoredView<AmpT> src;
+                src.a = amp;
+                return fused_probs(src, kind, fa);
+            }
         }
         return sweep_probs<BT>(amp, k, kind, fa, g, red);
     }
@@ -215,14 +253,14 @@ struct Interp {
         cands = 0;
     }
 
-    // svm.h:177-186
+    // svm.h:177-186 (pending sweeps first: expansions, then the run over them)
     CSVM_INLINE void scale_magnitude(double s) {
         g_re *= s;
         g_im *= s;
         double gm = hypot(g_re, g_im);
         if (gm > 1e100 || (gm < 1e-100 && gm > 0.0)) {
+            if (npend) flush_expand_oldest(npend);
             flush_mono();
-            flush_expand();
             sweep_scale<BT>(amp, k, gm, g);
             g_re /= gm;
             g_im /= gm;
@@ -605,10 +643,15 @@ struct Interp {
                 skip = a2;
                 continue;
             }
-            // an active measurement takes a run of two or more itself (probs)
-            if (nmono && !((kMonoKeepOps >> op) & 1ull) && !(nmono >= 2 && ((kMonoMeasOps >> op) & 1ull)))
-                flush_mono();
-            if (npend && !((kNoAmpOps >> op) & 1ull)) flush_expand();
+            // An active measurement takes a run of two or more itself
+            // (probs), and any run over pending expansions; with pend_defer
+            // the expansions stay pending under runs and up to that
+            // measurement.  Expansions are written before the run recorded
+            // over them.
+            const bool mflush = nmono && !((kMonoKeepOps >> op) & 1ull) && !((nmono >= 2 || npend) && ((kMonoMeasOps >> op) & 1ull));
+            if (npend && (mflush || (!((kNoAmpOps >> op) & 1ull) && !(pend_defer && (((kMonoOps | kMonoMeasOps) >> op) & 1ull)))))
+                flush_expand_oldest(npend);
+            if (mflush) flush_mono();
             switch (op) {
                 case CLIFFT_OP_FRAME_CNOT: frame_cnot(a1, a2); break;
                 case CLIFFT_OP_FRAME_CZ: frame_cz(a1, a2); break;
@@ -709,9 +752,10 @@ struct Interp {
                     }
                     const bool has_phase = op != CLIFFT_OP_EXPAND;
                     uint32_t px = f.gx(a1);
-                    // recorded now, written by flush_expand() together with
-                    // the expansions that follow it
-                    if (npend == kMaxFusedExpand) flush_expand();
+                    // recorded now, written by flush_expand_oldest() together
+                    // with the expansions that follow it (or read through by
+                    // a measurement pass)
+                    if (npend == kMaxFusedExpand) flush_expand_oldest(pend_defer ? 2u : kMaxFusedExpand);
                     if ((threadIdx.x & 63u) == 0) {
                         pend[2 * npend] = has_phase ? zr : 1.0;
                         pend[2 * npend + 1] = has_phase ? (px ? -zi : zi) : 0.0;
@@ -828,8 +872,8 @@ struct Interp {
                 default: break;
             }
         }
+        if (npend) flush_expand_oldest(npend);
         flush_mono();
-        flush_expand();
     }
 };
 

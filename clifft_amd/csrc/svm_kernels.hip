@@ -90,7 +90,9 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
     S.mops = nullptr;
     S.nmono = 0;
     S.cands = 0;
+    S.pend_defer = 0;
     if (GA && A->amp_slices >= 2) {  // wavefront teams: one list per wavefront
+        S.pend_defer = A->pend_defer ? 1u : 0u;
         S.amp_alt = (AmpT*)(gamp + amp_elems);
         if (A->amp_slices == 3) S.amp_alt2 = (AmpT*)(gamp + 2 * (size_t)amp_elems);
         S.mops = (lds_mono*)(p + (size_t)waves_block * rec_stride + (BT ? 0u : wave_in_block * kMonoBytes));

@@ -1,10 +1,10 @@
 // svm_sweeps.h -- amplitude sweeps and reductions of the team interpreter.
 //
-// Each sweep is a non-inlined device function taking only what it touches
-// (amplitude pointer in its own address space, rank, axes, coefficients),
-// so the dispatch loop keeps a small live register set.  AmpPtr is either an
-// LDS pointer (address space 3: ds_read/ds_write) or a global pointer
-// (address space 1: global_load/global_store).
+// Each sweep takes only what it touches (amplitude pointer in its own
+// address space, rank, axes, coefficients) and is inlined into the
+// interpreter.  AmpPtr is either an LDS pointer (address space 3:
+// ds_read/ds_write) or a global pointer (address space 1:
+// global_load/global_store).
 #pragma once
 
 #include "svm_device.h"
@@ -74,9 +74,9 @@ CSVM_INLINE void team_reduce(double (&v)[N], const TeamGeom& g, lds_f64* red) {
     for (int i = 0; i < N; ++i) v[i] = unid(v[i]);
 }
 
-// Arguments of non-inlined device functions arrive in vector registers;
-// re-assert their uniformity so loop control and coefficient loads stay
-// scalar.
+// Sweep arguments are re-asserted uniform (readfirstlane) so that loop
+// control and coefficient loads stay scalar whichever registers the caller
+// held them in.
 CSVM_INLINE TeamGeom uni_geom(TeamGeom g) {
     TeamGeom r;
     r.lane = g.lane;
@@ -339,13 +339,49 @@ CSVM_SWEEP void sweep_mono(const AmpT* src, AmpT* dst, uint32_t k, const lds_mon
 // Pair layouts of the active measurements' probability sums.
 enum : uint32_t { PROB_DIAG = 0, PROB_INTERFERE = 1, PROB_SWAP_INTERFERE = 2 };
 
+// Entry t 2^k0 + i of a state after m expansions b = 0 .. m-1 (phase z_b
+// when bit b of `mask` is set, else a plain copy) from entry i of the state
+// before them (t < 2^m): a times z_b for each set bit b of t, lowest first --
+// the products the one-at-a-time EXPAND sweeps form.
+CSVM_INLINE dv2 expand_entry(dv2 a, uint32_t t, const dv2 (&z)[kMaxFusedExpand], uint32_t mask) {
+#pragma unroll
+    for (uint32_t b = 0; b < kMaxFusedExpand; ++b)
+        if (((t >> b) & 1u) && ((mask >> b) & 1u)) a = cmul(a, z[b]);
+    return a;
+}
+
+// Read views of the state a fused measurement pass gathers from: the stored
+// entries, or the entries that m pending expansions of a stored state of
+// 2^k0 entries would write, computed on the fly by expand_entry, so the pass
+// reads 2^k0 entries instead of 2^(k0+m) and the expansions are never
+// written.
+template <class AmpT>
+struct StoredView {
+    const AmpT* a;
+    CSVM_INLINE dv2 operator()(uint32_t x) const { return a[x]; }
+};
+template <class AmpT>
+struct ExpandView {
+    const AmpT* a;
+    uint32_t k0, lo, m, mask;
+    dv2 z[kMaxFusedExpand];
+    CSVM_INLINE ExpandView(const AmpT* src, uint32_t k0_, uint32_t m_, const lds_f64* ph, uint32_t mask_)
+        : a(src), k0(uni(k0_)), lo((1u << uni(k0_)) - 1u), m(uni(m_)), mask(uni(mask_)) {
+#pragma unroll
+        for (uint32_t b = 0; b < kMaxFusedExpand; ++b)
+            z[b] = b < m ? mk(unid(ph[2 * b]), unid(ph[2 * b + 1])) : mk(1.0, 0.0);
+    }
+    CSVM_INLINE dv2 operator()(uint32_t x) const { return expand_entry(a[x & lo], x >> k0, z, mask); }
+};
+
 // A deferred permutation/phase run and the outcome probabilities of the
 // active measurement that follows it, in one pass.  dst receives the run's
 // result, entry for entry as sweep_mono writes it; the sums are sweep_probs'
 // over that result, each lane visiting the same pair indices in the same
-// order, so the measurement reads the state once instead of twice.
-template <bool BT, class AmpT>
-CSVM_SWEEP double2 sweep_mono_probs(const AmpT* src, AmpT* dst, uint32_t k, const lds_mono* ops, uint32_t n,
+// order, so the measurement reads the state once instead of twice.  `src`
+// is a StoredView or an ExpandView (pending expansions read through).
+template <bool BT, class View, class AmpT>
+CSVM_SWEEP double2 sweep_mono_probs(const View src, AmpT* dst, uint32_t k, const lds_mono* ops, uint32_t n,
                                     uint32_t kind, uint32_t fa, TeamGeom g, lds_f64* red) {
     k = uni(k); n = uni(n); kind = uni(kind); fa = uni(fa); g = uni_geom(g);
     const uint32_t half = 1u << (k - 1), t = k - 1, fb = 1u << fa;
@@ -379,8 +415,8 @@ CSVM_SWEEP double2 sweep_mono_probs(const AmpT* src, AmpT* dst, uint32_t k, cons
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (i0 + u * g.size < half) {
-                a[2 * u] = src[x[2 * u]];
-                a[2 * u + 1] = src[x[2 * u + 1]];
+                a[2 * u] = src(x[2 * u]);
+                a[2 * u + 1] = src(x[2 * u + 1]);
             }
         for (uint32_t j = 0; j < n; ++j) {
             const MonoOp o = load_mono(ops + j);
@@ -423,9 +459,10 @@ CSVM_SWEEP double2 sweep_mono_probs(const AmpT* src, AmpT* dst, uint32_t k, cons
 // stored run result, so the state kept is bit-identical to gathering,
 // measuring and folding in place; the sums are sweep_mono_probs'.  One read
 // and one write of 2^k entries replace the gather's read and write plus the
-// fold's read of 2^k and write of 2^(k-1).
-template <bool BT, class AmpT>
-CSVM_SWEEP double2 sweep_mono_cands(const AmpT* src, AmpT* c0, AmpT* c1, uint32_t k, const lds_mono* ops, uint32_t n,
+// fold's read of 2^k and write of 2^(k-1); through an ExpandView the read
+// is of the 2^(k-m) entries before m pending expansions.
+template <bool BT, class View, class AmpT>
+CSVM_SWEEP double2 sweep_mono_cands(const View src, AmpT* c0, AmpT* c1, uint32_t k, const lds_mono* ops, uint32_t n,
                                     uint32_t kind, uint32_t fa, TeamGeom g, lds_f64* red) {
     k = uni(k); n = uni(n); kind = uni(kind); fa = uni(fa); g = uni_geom(g);
     const uint32_t half = 1u << (k - 1), t = k - 1, fb = 1u << fa;
@@ -456,8 +493,8 @@ CSVM_SWEEP double2 sweep_mono_cands(const AmpT* src, AmpT* c0, AmpT* c1, uint32_
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (i0 + u * g.size < half) {
-                a[2 * u] = src[x[2 * u]];
-                a[2 * u + 1] = src[x[2 * u + 1]];
+                a[2 * u] = src(x[2 * u]);
+                a[2 * u + 1] = src(x[2 * u + 1]);
             }
         for (uint32_t j = 0; j < n; ++j) {
             const MonoOp o = load_mono(ops + j);
@@ -532,13 +569,7 @@ CSVM_SWEEP void sweep_expand_multi(AmpT* amp, uint32_t k0, uint32_t m, const lds
     const uint32_t K = 1u << k0, T = 1u << m;
     for (uint32_t i = g.lane; i < K; i += g.size) {
         const dv2 a = amp[i];
-        for (uint32_t t = 1; t < T; ++t) {
-            dv2 v = a;
-#pragma unroll
-            for (uint32_t b = 0; b < kMaxFusedExpand; ++b)
-                if (((t >> b) & 1u) && ((mask >> b) & 1u)) v = cmul(v, z[b]);
-            amp[i + t * K] = v;
-        }
+        for (uint32_t t = 1; t < T; ++t) amp[i + t * K] = expand_entry(a, t, z, mask);
     }
     team_sync<BT>();
 }

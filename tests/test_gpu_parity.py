@@ -243,3 +243,55 @@ def test_identity_meas_and_unequal_channels(svm, orc, nq, peak):
     assert np.array_equal(gpu < -1e300, ref < -1e300)
     ok = ref > -1e300
     np.testing.assert_allclose(gpu[ok], ref[ok], atol=1e-10, rtol=1e-12)
+
+
+def decay_program(peak, rounds, seed):
+    """Expansions to rank peak-1, then `rounds` rounds of: EXPAND of a new
+    top axis (exactly |+>), CNOTs from older axes into it (which leave |+>
+    unchanged) and CZs among the older axes -- a permutation/phase run over
+    a pending expansion -- then an interfere measurement of the new axis,
+    certain to give +.  Every round scales gamma by 1/sqrt 2 (and the
+    amplitudes by sqrt 2: the fold of |+>), so gamma falls below 1e-100
+    after about 660 rounds and the amplitudes are renormalised
+    (svm.h:177-186)."""
+    from clifft_amd.program import (Op, build_program, make_array_cnot, make_array_cz, make_expand,
+                                    make_expand_t, make_frame_h, make_meas)
+    rng = np.random.default_rng(seed)
+    bc = []
+    for v in range(peak - 1):
+        bc += [make_frame_h(v), make_expand_t(v)]
+    k = peak - 1
+    for r in range(rounds):
+        bc.append(make_expand(k))
+        for c in rng.choice(k, size=3, replace=False):
+            bc.append(make_array_cnot(int(c), k))
+        a, b = rng.choice(k, size=2, replace=False)
+        bc.append(make_array_cz(int(a), int(b)))
+        bc.append(make_meas(Op.MEAS_ACTIVE_INTERFERE, k, r))
+    return build_program(bc, peak_rank=peak, num_qubits=peak, num_measurements=rounds)
+
+
+@pytest.mark.parametrize("peak", [9, 14])
+def test_renormalisation(svm, orc, peak):
+    """Gamma renormalisation in the middle of a shot (LDS tier at peak 9,
+    global tier with pending expansions read through at peak 14): final
+    amplitudes (relative to their largest), gamma (relative) and records
+    against the oracle's execute, and sampled records."""
+    announce("renormalisation", peak)
+    rounds = 800
+    p = decay_program(peak, rounds, seed=peak)
+    o = orc.State.for_program(p, seed=5)
+    g = svm.State.for_program(p, seed=5)
+    svm.execute(p, g)
+    from oracle import oracle as orcmod
+    orcmod.execute(p, o)
+    assert g.active_k == o.active_k == peak - 1
+    n = 1 << g.active_k
+    scale = np.abs(o.v[:n]).max()
+    assert 0.0 < scale < 1e60  # rescaled: 800 folds alone would take them past 1e100
+    np.testing.assert_allclose(g.v[:n] / scale, o.v[:n] / scale, atol=ATOL, rtol=0)
+    assert abs(g.gamma / o.gamma - 1.0) < 1e-10
+    assert np.array_equal(g.meas_record, o.meas_record) and not np.asarray(o.meas_record[:rounds]).any()
+    r = svm.sample(p, 8, seed=3)
+    m, d, ob, e, _ = orc.sample(p, 8, seed=3, per_shot=True)
+    assert np.array_equal(r.measurements, m)
