@@ -437,7 +437,7 @@ def run_batch(args):
         shots = args.shots
     progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
     seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
-    r = _measure_batch(progs, shots, seeds0, args.steps, args.warmup, tune_shots=max(1, shots // 16),
+    r = _measure_batch(progs, shots, seeds0, args.steps, args.warmup, tune_shots=max(1, shots // 8),
                        batch_only=args.batch_only)
     total = shots * count
     value = total / r["t_batch"]
@@ -453,7 +453,8 @@ def run_batch(args):
         with open(tpath) as f:
             tr = json.load(f)
         w = tr["workloads"].get("batch4096_group_" + g["kind"])
-        if w and w["shots"] == shots and w.get("programs") == count:
+        # only for the group that was profiled (the tuning can move programs between groups)
+        if w and w["shots"] == shots and w.get("programs") == count and w.get("group_programs") == g["programs"]:
             traffic = (w["fetch_kib"] * tr["fetch_scale"] + w["write_kib"] * tr["write_scale"]) * 1024.0
     peaks = np.bincount([p.peak_rank for p in progs], minlength=17)
     line = {
@@ -582,6 +583,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "traffic_frac": None if traffic is None else traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "effective": achieved > HBM_PEAK_GBS,
+                     "limiter": ("HBM" if info["tier"] == 1 else
+                                 "instruction issue: amplitudes stay in LDS/registers, HBM carries the records"),
                      "measured_peaks": measured_peaks()},
         "backend": _lib.backend(),
     }

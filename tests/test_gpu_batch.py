@@ -108,7 +108,9 @@ def _snapshot(bufs):
     return [tuple(t.cpu().numpy().copy() for t in b) for b in bufs]
 
 
-TUNINGS = [dict(concurrent=0), dict(global_threads=512), dict(global_threads=1024), dict(lds_rank_max=8),
+TUNINGS = [dict(wave_rank_max=7, lds_rank_max=10, gwave_rank_max=14, wave_chunk=16, block_chunk=4, lane_chunk=256),
+           dict(gwave_rank_max=0), dict(concurrent=0), dict(global_threads=512), dict(global_threads=1024),
+           dict(lds_rank_max=8),
            dict(lds_rank_max=13), dict(wave_rank_max=5), dict(wave_rank_max=8, lds_rank_max=8),
            dict(wave_chunk=1, block_chunk=1), dict(wave_chunk=128, block_chunk=64),
            dict(global_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32),
@@ -198,13 +200,13 @@ def test_batch_graph_replays(mods, concurrent):
 def test_batch_groups_cover_every_shape(mods):
     """The mixed batch plans one launch group per kernel shape: lanes (by
     frame width), wavefront teams, LDS workgroup teams by size, global
-    wavefront teams (peak <= gwave_rank_max, 14 by default) and global
-    workgroup teams; every program sits in exactly one group."""
+    wavefront teams (peak <= gwave_rank_max) and global workgroup teams;
+    every program sits in exactly one group."""
     svm, batch = mods
     progs, shots, seeds = mixed_programs(61, 26, wide=True)
     b, bufs = _device_batch(batch, progs, shots)
     for gwave, kinds in ((14, {"lane", "wave", "lds", "gwave", "global"}), (0, {"lane", "wave", "lds", "global"})):
-        b.set_tuning(gwave_rank_max=gwave)
+        b.set_tuning(gwave_rank_max=gwave, wave_rank_max=7, lds_rank_max=10)
         groups = b.groups()
         assert {g["kind"] for g in groups} == kinds, gwave
         assert sum(g["programs"] for g in groups) == len(progs)

@@ -295,3 +295,20 @@ def test_renormalisation(svm, orc, peak):
     r = svm.sample(p, 8, seed=3)
     m, d, ob, e, _ = orc.sample(p, 8, seed=3, per_shot=True)
     assert np.array_equal(r.measurements, m)
+
+
+@pytest.mark.parametrize("peak,seed", [(12, 7), (14, 8)])
+def test_global_wavefront_teams(svm, orc, peak, seed):
+    """A launch with more shots than the device holds wavefront teams runs
+    global-amplitude shots one per wavefront: its first 200 shots (each
+    shot's stream depends only on the seed and the shot index) against the
+    oracle's 200-shot run."""
+    announce("global-wavefront-teams", peak, seed)
+    p = random_program(seed, num_qubits=peak + 6, peak=peak, length=200, p_array=0.5)
+    assert p.active_k_history().max() == peak
+    r = svm.sample(p, 4500, seed=11)
+    m, d, o, e, _ = orc.sample(p, 200, seed=11, per_shot=True)
+    assert np.array_equal(r.measurements[:200], m)
+    assert np.array_equal(r.detectors[:200], d)
+    assert np.array_equal(r.observables[:200], o)
+    np.testing.assert_allclose(r.exp_vals[:200], e, atol=ATOL, rtol=0)
