@@ -189,12 +189,21 @@ constexpr uint64_t kMonoMeasOps =
 
 // One deferred permutation/phase opcode of a fused run (global tier): the
 // run's opcodes are applied to every amplitude in one out-of-place pass.
+// The pass applies each in one branch-free form, to an entry at index x:
+//   phase    when x & qa is nonzero and popc(x & qm) is odd, pk 1 negates
+//            the entry and pk 2 multiplies it by (pr, pi) (pk 0: none);
+//   permute  x ^= px when popc(x & pm) is odd (pm 0: none).
+// CNOT(c, t): pm = c, px = t.  SWAP(a, b): pm = px = a | b (flips both bits
+// when they differ).  MULTI_CNOT(t, m): pm = m, px = t.  CZ(a, b): qa = a,
+// qm = b, pk 1.  MULTI_CZ(a, m): qa = a, qm = m, pk 1.  S/S_DAG/T/T_DAG/ROT
+// on a: qa = qm = a, pk 2 (masks as single bits).  kind, a and b name the
+// opcode for the single-opcode sweeps.
 enum : uint32_t { MONO_CNOT = 0, MONO_CZ = 1, MONO_SWAP = 2, MONO_MULTI_CNOT = 3, MONO_MULTI_CZ = 4, MONO_PHASE = 5 };
 struct MonoOp {
     uint32_t kind;
     uint32_t a;  // control / first axis / MULTI axis / phase axis
     uint32_t b;  // target / second axis
-    uint32_t m;  // MULTI_* mask (active axes only)
+    uint32_t pm, px, qa, qm, pk;
     double pr, pi;  // MONO_PHASE factor
 };
 constexpr uint32_t kMaxMono = 32;

@@ -171,9 +171,25 @@ struct Interp {
             o.kind = kind;
             o.a = a;
             o.b = b;
-            o.m = m;
+            o.pm = o.px = o.qa = o.qm = o.pk = 0u;
             o.pr = pr;
             o.pi = pi;
+            if (kind == MONO_CNOT) {
+                o.pm = 1u << a;
+                o.px = 1u << b;
+            } else if (kind == MONO_SWAP) {
+                o.pm = o.px = (1u << a) | (1u << b);
+            } else if (kind == MONO_MULTI_CNOT) {
+                o.pm = m;
+                o.px = 1u << a;
+            } else if (kind == MONO_CZ || kind == MONO_MULTI_CZ) {
+                o.qa = 1u << a;
+                o.qm = kind == MONO_CZ ? 1u << b : m;
+                o.pk = 1u;
+            } else {  // MONO_PHASE
+                o.qa = o.qm = 1u << a;
+                o.pk = 2u;
+            }
             store_mono(mops + nmono, o);
         }
         ++nmono;
@@ -186,14 +202,13 @@ struct Interp {
         if (!kGlobalAmps || nmono == 0) return;
         team_sync<BT>();  // the list written by lane 0 is visible to every wave
         if (nmono == 1) {
-            const MonoOp o = load_mono(mops);
-            const uint32_t kind = uni(o.kind);
-            if (kind == MONO_CNOT) sweep_two_axis<BT>(amp, k, o.a, o.b, PERM_CNOT, g);
-            else if (kind == MONO_CZ) sweep_two_axis<BT>(amp, k, o.a, o.b, PERM_CZ, g);
-            else if (kind == MONO_SWAP) sweep_two_axis<BT>(amp, k, o.a, o.b, PERM_SWAP, g);
-            else if (kind == MONO_MULTI_CNOT) sweep_multi<BT>(amp, k, o.a, o.m, 0u, g);
-            else if (kind == MONO_MULTI_CZ) sweep_multi<BT>(amp, k, o.a, o.m, 1u, g);
-            else sweep_phase<BT>(amp, k, o.a, o.pr, o.pi, g);
+            const uint32_t kind = uni(mops->kind), a = uni(mops->a), b = uni(mops->b);
+            if (kind == MONO_CNOT) sweep_two_axis<BT>(amp, k, a, b, PERM_CNOT, g);
+            else if (kind == MONO_CZ) sweep_two_axis<BT>(amp, k, a, b, PERM_CZ, g);
+            else if (kind == MONO_SWAP) sweep_two_axis<BT>(amp, k, a, b, PERM_SWAP, g);
+            else if (kind == MONO_MULTI_CNOT) sweep_multi<BT>(amp, k, a, uni(mops->pm), 0u, g);
+            else if (kind == MONO_MULTI_CZ) sweep_multi<BT>(amp, k, a, uni(mops->qm), 1u, g);
+            else sweep_phase<BT>(amp, k, a, unid(mops->pr), unid(mops->pi), g);
         } else {
             sweep_mono<BT>(amp, amp_alt, k, mops, nmono, g);
             AmpT* t = amp;

@@ -269,38 +269,55 @@ CSVM_SWEEP void sweep_u4(AmpT* amp, uint32_t k, uint32_t lo, uint32_t hi, cptr<d
 // products in the same order, so the values are bit-identical).  Each index
 // stays below 2^k: the opcodes only exchange or flip bits below k.
 typedef __attribute__((address_space(3))) MonoOp lds_mono;
-CSVM_INLINE MonoOp load_mono(const lds_mono* p) {
-    MonoOp o;
-    o.kind = p->kind;
-    o.a = p->a;
-    o.b = p->b;
-    o.m = p->m;
-    o.pr = p->pr;
-    o.pi = p->pi;
-    return o;
-}
 CSVM_INLINE void store_mono(lds_mono* p, const MonoOp& o) {
     p->kind = o.kind;
     p->a = o.a;
     p->b = o.b;
-    p->m = o.m;
+    p->pm = o.pm;
+    p->px = o.px;
+    p->qa = o.qa;
+    p->qm = o.qm;
+    p->pk = o.pk;
     p->pr = o.pr;
     p->pi = o.pi;
 }
-CSVM_INLINE uint32_t mono_perm(const MonoOp& o, uint32_t x) {
-    if (o.kind == MONO_CNOT) return x ^ (((x >> o.a) & 1u) << o.b);
-    if (o.kind == MONO_SWAP) {
-        uint32_t d = ((x >> o.a) ^ (x >> o.b)) & 1u;
-        return x ^ ((d << o.a) | (d << o.b));
+// The run's permutations undone in reverse order (each is its own inverse):
+// the source index of each of the N entries.  One uniform test per opcode;
+// the entries' work is branch-free.
+template <uint32_t N>
+CSVM_INLINE void mono_sources(const lds_mono* ops, uint32_t n, uint32_t (&x)[N]) {
+    for (uint32_t j = n; j-- > 0;) {
+        const uint32_t pm = uni(ops[j].pm), px = uni(ops[j].px);
+        if (pm) {
+#pragma unroll
+            for (uint32_t e = 0; e < N; ++e) x[e] = (__popc(x[e] & pm) & 1u) ? x[e] ^ px : x[e];
+        }
     }
-    if (o.kind == MONO_MULTI_CNOT) return x ^ (((uint32_t)__popc(x & o.m) & 1u) << o.a);
-    return x;
 }
-CSVM_INLINE dv2 mono_phase(const MonoOp& o, uint32_t x, dv2 v) {
-    if (o.kind == MONO_CZ) return ((x >> o.a) & (x >> o.b) & 1u) ? cneg(v) : v;
-    if (o.kind == MONO_MULTI_CZ) return (((x >> o.a) & 1u) && (__popc(x & o.m) & 1u)) ? cneg(v) : v;
-    if (o.kind == MONO_PHASE) return ((x >> o.a) & 1u) ? cmul(v, mk(o.pr, o.pi)) : v;
-    return v;
+// The entries walked forward through the run: each opcode's phase at the
+// index the entry holds at that point, then its permutation.
+template <uint32_t N>
+CSVM_INLINE void mono_walk(const lds_mono* ops, uint32_t n, uint32_t (&x)[N], dv2 (&v)[N]) {
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t pk = uni(ops[j].pk), pm = uni(ops[j].pm), px = uni(ops[j].px);
+        if (pk) {
+            const uint32_t qa = uni(ops[j].qa), qm = uni(ops[j].qm);
+            if (pk == 1) {
+#pragma unroll
+                for (uint32_t e = 0; e < N; ++e)
+                    if ((x[e] & qa) && (__popc(x[e] & qm) & 1u)) v[e] = cneg(v[e]);
+            } else {
+                const dv2 ph = mk(unid(ops[j].pr), unid(ops[j].pi));
+#pragma unroll
+                for (uint32_t e = 0; e < N; ++e)
+                    if ((x[e] & qa) && (__popc(x[e] & qm) & 1u)) v[e] = cmul(v[e], ph);
+            }
+        }
+        if (pm) {
+#pragma unroll
+            for (uint32_t e = 0; e < N; ++e) x[e] = (__popc(x[e] & pm) & 1u) ? x[e] ^ px : x[e];
+        }
+    }
 }
 template <bool BT, class AmpT>
 CSVM_SWEEP void sweep_mono(const AmpT* src, AmpT* dst, uint32_t k, const lds_mono* ops, uint32_t n, TeamGeom g) {
@@ -313,22 +330,11 @@ CSVM_SWEEP void sweep_mono(const AmpT* src, AmpT* dst, uint32_t k, const lds_mon
         dv2 v[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) x[u] = y0 + u * g.size;
-        for (uint32_t j = n; j-- > 0;) {
-            const MonoOp o = load_mono(ops + j);
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u) x[u] = mono_perm(o, x[u]);
-        }
+        mono_sources(ops, n, x);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (y0 + u * g.size < K) v[u] = src[x[u]];
-        for (uint32_t j = 0; j < n; ++j) {
-            const MonoOp o = load_mono(ops + j);
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u) {
-                v[u] = mono_phase(o, x[u], v[u]);
-                x[u] = mono_perm(o, x[u]);
-            }
-        }
+        mono_walk(ops, n, x, v);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (y0 + u * g.size < K) dst[y0 + u * g.size] = v[u];
@@ -407,25 +413,14 @@ CSVM_SWEEP double2 sweep_mono_probs(const View src, AmpT* dst, uint32_t k, const
             x[2 * u] = y[2 * u];
             x[2 * u + 1] = y[2 * u + 1];
         }
-        for (uint32_t j = n; j-- > 0;) {
-            const MonoOp o = load_mono(ops + j);
-#pragma unroll
-            for (uint32_t e = 0; e < 2 * U; ++e) x[e] = mono_perm(o, x[e]);
-        }
+        mono_sources(ops, n, x);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (i0 + u * g.size < half) {
                 a[2 * u] = src(x[2 * u]);
                 a[2 * u + 1] = src(x[2 * u + 1]);
             }
-        for (uint32_t j = 0; j < n; ++j) {
-            const MonoOp o = load_mono(ops + j);
-#pragma unroll
-            for (uint32_t e = 0; e < 2 * U; ++e) {
-                a[e] = mono_phase(o, x[e], a[e]);
-                x[e] = mono_perm(o, x[e]);
-            }
-        }
+        mono_walk(ops, n, x, a);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (i0 + u * g.size < half) {
@@ -485,25 +480,14 @@ CSVM_SWEEP double2 sweep_mono_cands(const View src, AmpT* c0, AmpT* c1, uint32_t
             x[2 * u] = y[2 * u];
             x[2 * u + 1] = y[2 * u + 1];
         }
-        for (uint32_t j = n; j-- > 0;) {
-            const MonoOp o = load_mono(ops + j);
-#pragma unroll
-            for (uint32_t e = 0; e < 2 * U; ++e) x[e] = mono_perm(o, x[e]);
-        }
+        mono_sources(ops, n, x);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (i0 + u * g.size < half) {
                 a[2 * u] = src(x[2 * u]);
                 a[2 * u + 1] = src(x[2 * u + 1]);
             }
-        for (uint32_t j = 0; j < n; ++j) {
-            const MonoOp o = load_mono(ops + j);
-#pragma unroll
-            for (uint32_t e = 0; e < 2 * U; ++e) {
-                a[e] = mono_phase(o, x[e], a[e]);
-                x[e] = mono_perm(o, x[e]);
-            }
-        }
+        mono_walk(ops, n, x, a);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             const uint32_t i = i0 + u * g.size;
