@@ -14,15 +14,16 @@ workloads time one program's sampling launch (surface_d3, clifford_t20,
 large_k34, clifford_t24_k16), the per-program launches alone (programs4096)
 or the adjoint gradient (rz24_grad).
 
-roofline: algorithmic bytes of the dominant kernel's launch (per-shot
-state-traffic model: 16 B per amplitude read or written by each array opcode
-at its static rank, plus the record bytes written) / that kernel's duration
-from HIP events on the stream it runs on, against the 8 TB/s HBM peak; when
-fused passes move fewer bytes than the model charges, achieved can pass the
-peak and "effective" is set.  `traffic` is the HBM bytes per launch measured
-with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (profiles/hbm_traffic.json,
-scaled by a copy-kernel calibration), with traffic_frac = traffic / duration
-/ peak, or null when no measurement exists for the workload.
+roofline: HBM bytes of the dominant kernel's launch / that kernel's duration
+from HIP events on the stream it runs on, against the 8 TB/s HBM peak.  The
+bytes (`traffic`) are the ones measured with rocprofv3 FETCH_SIZE /
+WRITE_SIZE counters for that kernel and workload (profiles/hbm_traffic.json,
+scaled by a copy-kernel calibration; "achieved_from": "PMC traffic").  The
+per-shot byte model (16 B per amplitude read or written by each array opcode
+at its static rank, plus the record bytes) rides along as
+algorithmic_bytes_per_launch / model_GBps; fused passes move fewer bytes
+than it charges ("effective": model above the peak).  Without a measurement
+for the kernel, the model stands in ("achieved_from": "byte model").
 cpu_baseline: the C oracle built with -O3 -ffast-math -fopenmp, single PRNG
 stream, shots in sequence (the reference's scheduling), on a bounded sample;
 the shot-parallel OpenMP figure on all host cores rides along as "mode_b".
@@ -237,12 +238,26 @@ def run_grad(args):
                    "max_abs_diff_vs_param_shift": float(np.abs(G - Gps).max()),
                    "max_abs_diff_value": float(np.abs(F - Fps[:, 0]).max()),
                    "param_shift_kernel_ms": kern_ps, "parallelism": "theta vectors"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kern_ms,
-                     "algorithmic_bytes_per_launch": alg_vec * B},
+        "roofline": _roofline(achieved, None, kern_ms, alg_vec * B),
         "backend": _lib.backend(),
     }
     print(json.dumps(line), flush=True)
+
+
+def _roofline(model_gbs, traffic, kernel_ms, model_bytes, **extra):
+    """HBM roofline of the dominant kernel.  `achieved` and `frac` come from
+    the PMC-measured HBM bytes of that kernel (profiles/hbm_traffic.json)
+    when they were profiled for it; the byte model (one sweep per opcode,
+    an upper bound: fused passes move fewer bytes) is kept as
+    `model_GBps` and stands in, flagged, only without a measurement."""
+    measured = traffic is not None
+    achieved = traffic / (kernel_ms * 1e-3) / 1e9 if measured else model_gbs
+    r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+         "traffic": traffic, "achieved_from": "PMC traffic" if measured else "byte model",
+         "kernel_ms": kernel_ms, "algorithmic_bytes_per_launch": model_bytes, "model_GBps": model_gbs,
+         "effective": model_gbs > HBM_PEAK_GBS}
+    r.update(extra)
+    return r
 
 
 def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots, batch_only=False):
@@ -414,9 +429,7 @@ def run_programs(args):
         "config": {"workload": "programs4096", "description": desc, "programs": count, "shots_per_program": shots,
                    "peak_rank_histogram": peaks.tolist(), "records": "bit-packed measurements, device memory",
                    "parallelism": "shots within each program"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": wall * 1e3,
-                     "algorithmic_bytes_per_launch": alg, "note": "time is wall time of all launches of one step"},
+        "roofline": _roofline(achieved, None, wall * 1e3, alg, note="time is wall time of all launches of one step"),
         "backend": _lib.backend(),
     }
     print(json.dumps(line), flush=True)
@@ -471,12 +484,8 @@ def run_batch(args):
                    "graph_replay_shots_per_s": _num(total / r["t_graph"]),
                    "speedup_graph_vs_per_program": _num(r["t_per_program"] / r["t_graph"]),
                    "parallelism": "programs x shots"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"launch group {gi} ({g['kind']}, {g['programs']} programs, block {g['block']})",
-                     "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes,
-                     "traffic_frac": None if traffic is None else traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "effective": achieved > HBM_PEAK_GBS},
+        "roofline": _roofline(achieved, traffic, dom_ms, dom_bytes,
+                              kernel=f"launch group {gi} ({g['kind']}, {g['programs']} programs, block {g['block']})"),
         "backend": _lib.backend(),
     }
     if not args.no_cpu:
@@ -580,14 +589,10 @@ def main():
                    "tier": ["team-lds", "team-global", "lane"][info["tier"]], "team_waves": info["team_waves"],
                    "shots_per_block": info["shots_per_block"], "records": "bit-packed, device memory",
                    "parallelism": "shots"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "traffic_frac": None if traffic is None else traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "effective": achieved > HBM_PEAK_GBS,
-                     "limiter": ("HBM" if info["tier"] == 1 else
-                                 "instruction issue: amplitudes stay in LDS/registers, HBM carries the records"),
-                     "measured_peaks": measured_peaks()},
+        "roofline": _roofline(achieved, traffic, kern_ms, bytes_per_launch,
+                              limiter=("HBM" if info["tier"] == 1 else
+                                       "instruction issue: amplitudes stay in LDS/registers, HBM carries the records"),
+                              measured_peaks=measured_peaks()),
         "backend": _lib.backend(),
     }
     if not args.no_cpu:
