@@ -55,3 +55,31 @@ def test_large_k34_bit_exact(svm, orc, final_measure, shots):
     assert np.array_equal(res.measurements, m)
     assert np.array_equal(res.detectors, d)
     assert np.array_equal(res.observables, o)
+
+
+def test_large_k34_graph_replay(svm, orc):
+    """Per-program HIP graph replay of the large-rank workload: a
+    one-program batch captured once and replayed with new seeds, with eager
+    work between replays; every replay equals an eager launch with the same
+    seed, and the seed-42 launch the oracle's per-shot records."""
+    import torch
+    from clifft_amd import batch
+    announce("large-k34-graph")
+    p = large_k_program(True)
+    shots = 2
+    m = torch.zeros((shots, (p.num_measurements + 7) // 8), dtype=torch.uint8, device="cuda:0")
+    b = batch.Batch([p], [shots], [{"measurements": m.data_ptr()}], packed=True)
+    ref = np.packbits(orc.sample(p, shots, seed=42, per_shot=True)[0], axis=1, bitorder="little")
+    for seed in (42, 7, 9):
+        b.run([seed])
+        torch.cuda.synchronize()
+        eager = m.cpu().numpy().copy()
+        m.fill_(255)
+        torch.cuda.synchronize()
+        b.run_graph([seed])
+        torch.cuda.synchronize()
+        assert np.array_equal(m.cpu().numpy(), eager), seed
+        if seed == 42:
+            assert np.array_equal(eager, ref)
+        _ = torch.ones(1000, device="cuda:0").sum().item()  # unrelated eager work between replays
+    b.close()
