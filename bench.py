@@ -260,7 +260,7 @@ def _roofline(model_gbs, traffic, kernel_ms, model_bytes, **extra):
     return r
 
 
-def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots, batch_only=False):
+def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots, batch_only=False, fixed_tuning=None):
     """The batch (autotuned on a `tune_shots`-per-program sample) against one
     plain launch per program, on the same stream and outputs.  Returns
     timings in seconds and the per-group kernel times of one eager run with
@@ -285,6 +285,8 @@ def _measure_batch(progs, shots, seeds0, steps, warmup, tune_shots, batch_only=F
     sh = stream.cuda_stream
     bt = B.Batch(progs, [shots] * count, outs, packed=True)
     t_tune = time.perf_counter()
+    if batch_only and fixed_tuning:
+        bt.set_tuning(**fixed_tuning)
     tuning = bt.tuning if batch_only else bt.tune(sample_shots=tune_shots, stream=sh)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t_tune
@@ -451,7 +453,8 @@ def run_batch(args):
     progs = [ccompile(text) for text, _ in W.batch_programs(count=count, seed=42, shots=shots)]
     seeds0 = np.array([42 ^ i for i in range(count)], dtype=np.uint64)
     r = _measure_batch(progs, shots, seeds0, args.steps, args.warmup, tune_shots=max(1, shots // 8),
-                       batch_only=args.batch_only)
+                       batch_only=args.batch_only,
+                       fixed_tuning=json.loads(args.batch_tuning) if args.batch_tuning else None)
     total = shots * count
     value = total / r["t_batch"]
     # dominant kernel: the launch group with the longest kernel time
@@ -465,8 +468,8 @@ def run_batch(args):
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        w = tr["workloads"].get("batch4096_group_" + g["kind"])
-        # only for the group that was profiled (the tuning can move programs between groups)
+        # profiled per group membership (the tuning moves programs between groups)
+        w = tr["workloads"].get(f"batch4096_group_{g['kind']}_{g['programs']}")
         if w and w["shots"] == shots and w.get("programs") == count and w.get("group_programs") == g["programs"]:
             traffic = (w["fetch_kib"] * tr["fetch_scale"] + w["write_kib"] * tr["write_scale"]) * 1024.0
     peaks = np.bincount([p.peak_rank for p in progs], minlength=17)
@@ -503,6 +506,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch4096: default tuning, no per-program launches or graph replays (to profile)")
+    ap.add_argument("--batch-tuning", default="", metavar="JSON",
+                    help="with --batch-only: tuning fields over the default, e.g. '{\"wave_rank_max\": 9}'")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()

@@ -1634,8 +1634,8 @@ clifft_svm_batch_tuning default_tuning() {
 int batch_check_tuning(const clifft_svm_batch_tuning& t) {
     if (t.global_threads != 256 && t.global_threads != 512 && t.global_threads != 1024)
         return fail(CLIFFT_E_INVALID, "batch tuning: global_threads must be 256, 512 or 1024");
-    if (t.wave_rank_max > 8 || t.lds_rank_max > 13 || t.lds_rank_max < t.wave_rank_max)
-        return fail(CLIFFT_E_INVALID, "batch tuning: need wave_rank_max <= lds_rank_max, wave_rank_max <= 8, lds_rank_max <= 13");
+    if (t.wave_rank_max > 10 || t.lds_rank_max > 13 || t.lds_rank_max < t.wave_rank_max)
+        return fail(CLIFFT_E_INVALID, "batch tuning: need wave_rank_max <= lds_rank_max, wave_rank_max <= 10, lds_rank_max <= 13");
     if (t.wave_chunk == 0 || t.block_chunk == 0 || t.lane_chunk == 0)
         return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
     if (t.lane_rank_max > 6) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 6");
@@ -2347,8 +2347,28 @@ int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stre
     if (!rc) try_u32(&clifft_svm_batch_tuning::concurrent, {0});
     if (!rc) try_lane({-1, 4, 5});
     if (!rc) try_u32(&clifft_svm_batch_tuning::lane_chunk, {64, 256, 512, 1024});
-    if (!rc) try_u32(&clifft_svm_batch_tuning::lds_rank_max, {9, 10, 11, 12});
-    if (!rc) try_u32(&clifft_svm_batch_tuning::wave_rank_max, {6, 7, 8});
+    // the LDS thresholds together: which peaks run on wavefront teams in LDS,
+    // on workgroup teams in LDS, and (above both) on global amplitudes
+    auto try_lds = [&](std::initializer_list<std::pair<uint32_t, uint32_t>> pairs) {
+        for (const auto& wl : pairs) {
+            if (rc) return;
+            clifft_svm_batch_tuning t = best;
+            t.wave_rank_max = wl.first;
+            t.lds_rank_max = wl.second;
+            if (batch_check_tuning(t) != CLIFFT_OK) continue;
+            float ms = 0.f;
+            int r = time_of(t, &ms);
+            if (r == CLIFFT_E_HIP) {
+                rc = r;
+                return;
+            }
+            if (r == CLIFFT_OK && ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    };
+    if (!rc) try_lds({{6, 6}, {7, 7}, {9, 9}, {10, 10}, {7, 8}, {8, 10}, {8, 12}});
     if (!rc) try_u32(&clifft_svm_batch_tuning::gwave_rank_max, {12, 13, 14, 15, 16, 18});
     if (!rc) try_u32(&clifft_svm_batch_tuning::global_threads, {512, 1024});
     if (!rc) try_u32(&clifft_svm_batch_tuning::wave_chunk, {2, 8, 16, 32});

@@ -389,7 +389,7 @@ typedef struct clifft_svm_batch clifft_svm_batch;
 
 typedef struct {
     uint32_t global_threads; /* 256, 512 or 1024: team size of the global-amplitude group */
-    uint32_t wave_rank_max;  /* <= 8 */
+    uint32_t wave_rank_max;  /* <= 10 (four LDS states per workgroup) */
     uint32_t lds_rank_max;   /* <= 13, >= wave_rank_max */
     uint32_t wave_chunk;     /* shots per work item, wavefront teams */
     uint32_t block_chunk;    /* shots per work item, workgroup teams */

@@ -116,7 +116,8 @@ TUNINGS = [dict(wave_rank_max=7, lds_rank_max=10, gwave_rank_max=14, wave_chunk=
            dict(global_threads=512, wave_rank_max=6, lds_rank_max=12, wave_chunk=128, block_chunk=32),
            dict(lane_rank_max=-1), dict(lane_rank_max=5, lane_chunk=100), dict(lane_rank_max=1, lane_chunk=64),
            dict(lane_rank_max=6, lane_chunk=1024), dict(concurrent=0, lane_rank_max=4, lane_chunk=3000),
-           dict(gwave_rank_max=13), dict(gwave_rank_max=16, wave_chunk=3), dict(gwave_rank_max=30, lds_rank_max=8)]
+           dict(gwave_rank_max=13), dict(gwave_rank_max=16, wave_chunk=3), dict(gwave_rank_max=30, lds_rank_max=8),
+           dict(wave_rank_max=10, lds_rank_max=10), dict(wave_rank_max=6, lds_rank_max=6, gwave_rank_max=16)]
 
 
 def test_batch_tunings_agree(mods):
@@ -256,7 +257,7 @@ def test_batch_rejects_bad_tuning(mods):
     with pytest.raises(SvmError):
         b.set_tuning(concurrent=2)
     with pytest.raises(SvmError):
-        b.set_tuning(wave_rank_max=9)
+        b.set_tuning(wave_rank_max=11, lds_rank_max=12)
     with pytest.raises(SvmError):
         b.set_tuning(lds_rank_max=14)
     with pytest.raises(SvmError):
