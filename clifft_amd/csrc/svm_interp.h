@@ -219,10 +219,11 @@ struct Interp {
     }
 
     // Outcome probabilities of an active measurement.  A deferred run of two
-    // or more permutation/phase opcodes is applied in the same pass (into the
-    // other slice) instead of a gather of its own followed by a reading pass;
-    // pending expansions under it (or under no run) are read through, so the
-    // pass reads the state from before them.
+    // or more permutation/phase opcodes (any run, with a third slice) is
+    // applied in the same pass (into the other slice) instead of a gather of
+    // its own followed by a reading pass; pending expansions under it (or
+    // under no run) are read through, so the pass reads the state from
+    // before them.
     template <class View>
     CSVM_INLINE double2 fused_probs(const View& src, uint32_t kind, uint32_t fa) {
         double2 pr;
@@ -247,7 +248,10 @@ struct Interp {
                 team_sync<BT>();  // the list written by lane 0 is visible to every wave
                 return fused_probs(ExpandView<AmpT>(amp, k - npend, npend, pend, pend_mask), kind, fa);
             }
-            if (nmono >= 2) {
+            // with a third slice every active measurement folds in its pass
+            // (a run of zero or one opcode too: one read and one write of
+            // the state instead of the opcode's sweep, the sums and the fold)
+            if (nmono >= 2 || amp_alt2) {
                 team_sync<BT>();
                 StoredView<AmpT> src;
                 src.a = amp;
@@ -663,7 +667,8 @@ struct Interp {
             // the expansions stay pending under runs and up to that
             // measurement.  Expansions are written before the run recorded
             // over them.
-            const bool mflush = nmono && !((kMonoKeepOps >> op) & 1ull) && !((nmono >= 2 || npend) && ((kMonoMeasOps >> op) & 1ull));
+            const bool mflush = nmono && !((kMonoKeepOps >> op) & 1ull) &&
+                                !((nmono >= 2 || npend || amp_alt2) && ((kMonoMeasOps >> op) & 1ull));
             if (npend && (mflush || (!((kNoAmpOps >> op) & 1ull) && !(pend_defer && (((kMonoOps | kMonoMeasOps) >> op) & 1ull)))))
                 flush_expand_oldest(npend);
             if (mflush) flush_mono();

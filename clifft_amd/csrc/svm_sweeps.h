@@ -390,7 +390,9 @@ template <bool BT, class View, class AmpT>
 CSVM_SWEEP double2 sweep_mono_probs(const View src, AmpT* dst, uint32_t k, const lds_mono* ops, uint32_t n,
                                     uint32_t kind, uint32_t fa, TeamGeom g, lds_f64* red) {
     k = uni(k); n = uni(n); kind = uni(kind); fa = uni(fa); g = uni_geom(g);
-    const uint32_t half = 1u << (k - 1), t = k - 1, fb = 1u << fa;
+    const uint32_t half = 1u << (k - 1), t = k - 1;
+    // pair axis: fa for SWAP_MEAS_INTERFERE, else the top axis t
+    const uint32_t fs = kind == PROB_SWAP_INTERFERE ? fa : t, fb = 1u << fs;
     // pairs in flight per lane: as many loads as sweep_mono keeps
     constexpr uint32_t U = Unroll<AmpT>::value > 1 ? Unroll<AmpT>::value / 2 : 1;
     double v[2] = {0.0, 0.0};
@@ -400,16 +402,10 @@ CSVM_SWEEP double2 sweep_mono_probs(const View src, AmpT* dst, uint32_t k, const
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             // the pair sweep_probs reads for index i (in range or not,
-            // y stays below 2^k)
+            // y stays below 2^k); with fs = k-1 this is (i, i + half)
             const uint32_t i = (i0 + u * g.size) & (half - 1u);
-            if (kind == PROB_SWAP_INTERFERE) {
-                const uint32_t bf = (i >> fa) & 1u;
-                y[2 * u] = (i & ~fb) | (bf << t);
-                y[2 * u + 1] = y[2 * u] | fb;
-            } else {
-                y[2 * u] = i;
-                y[2 * u + 1] = i + half;
-            }
+            y[2 * u] = (i & ~fb) | (((i >> fs) & 1u) << t);
+            y[2 * u + 1] = y[2 * u] | fb;
             x[2 * u] = y[2 * u];
             x[2 * u + 1] = y[2 * u + 1];
         }
@@ -421,21 +417,28 @@ CSVM_SWEEP double2 sweep_mono_probs(const View src, AmpT* dst, uint32_t k, const
                 a[2 * u + 1] = src(x[2 * u + 1]);
             }
         mono_walk(ops, n, x, a);
+        if (kind == PROB_DIAG) {
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            if (i0 + u * g.size < half) {
-                const dv2 p = a[2 * u], q = a[2 * u + 1];
-                dst[y[2 * u]] = p;
-                dst[y[2 * u + 1]] = q;
-                if (kind == PROB_DIAG) {
+            for (uint32_t u = 0; u < U; ++u)
+                if (i0 + u * g.size < half) {
+                    const dv2 p = a[2 * u], q = a[2 * u + 1];
+                    dst[y[2 * u]] = p;
+                    dst[y[2 * u + 1]] = q;
                     v[0] += p.x * p.x + p.y * p.y;
                     v[1] += q.x * q.x + q.y * q.y;
-                } else {
+                }
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                if (i0 + u * g.size < half) {
+                    const dv2 p = a[2 * u], q = a[2 * u + 1];
+                    dst[y[2 * u]] = p;
+                    dst[y[2 * u + 1]] = q;
                     const dv2 s = cadd(p, q), d = csub(p, q);
                     v[0] += s.x * s.x + s.y * s.y;
                     v[1] += d.x * d.x + d.y * d.y;
                 }
-            }
+        }
     }
     team_sync<BT>();
     team_reduce<BT, 2>(v, g, red);
@@ -460,7 +463,9 @@ template <bool BT, class View, class AmpT>
 CSVM_SWEEP double2 sweep_mono_cands(const View src, AmpT* c0, AmpT* c1, uint32_t k, const lds_mono* ops, uint32_t n,
                                     uint32_t kind, uint32_t fa, TeamGeom g, lds_f64* red) {
     k = uni(k); n = uni(n); kind = uni(kind); fa = uni(fa); g = uni_geom(g);
-    const uint32_t half = 1u << (k - 1), t = k - 1, fb = 1u << fa;
+    const uint32_t half = 1u << (k - 1), t = k - 1;
+    // pair axis: fa for SWAP_MEAS_INTERFERE, else the top axis t
+    const uint32_t fs = kind == PROB_SWAP_INTERFERE ? fa : t, fb = 1u << fs;
     constexpr uint32_t U = Unroll<AmpT>::value > 1 ? Unroll<AmpT>::value / 2 : 1;
     double v[2] = {0.0, 0.0};
     for (uint32_t i0 = g.lane; i0 < half; i0 += U * g.size) {
@@ -469,14 +474,8 @@ CSVM_SWEEP double2 sweep_mono_cands(const View src, AmpT* c0, AmpT* c1, uint32_t
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             const uint32_t i = (i0 + u * g.size) & (half - 1u);
-            if (kind == PROB_SWAP_INTERFERE) {
-                const uint32_t bf = (i >> fa) & 1u;
-                y[2 * u] = (i & ~fb) | (bf << t);
-                y[2 * u + 1] = y[2 * u] | fb;
-            } else {
-                y[2 * u] = i;
-                y[2 * u + 1] = i + half;
-            }
+            y[2 * u] = (i & ~fb) | (((i >> fs) & 1u) << t);
+            y[2 * u + 1] = y[2 * u] | fb;
             x[2 * u] = y[2 * u];
             x[2 * u + 1] = y[2 * u + 1];
         }
@@ -488,17 +487,24 @@ CSVM_SWEEP double2 sweep_mono_cands(const View src, AmpT* c0, AmpT* c1, uint32_t
                 a[2 * u + 1] = src(x[2 * u + 1]);
             }
         mono_walk(ops, n, x, a);
+        if (kind == PROB_DIAG) {
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t i = i0 + u * g.size;
-            if (i < half) {
-                const dv2 p = a[2 * u], q = a[2 * u + 1];
-                if (kind == PROB_DIAG) {
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u * g.size;
+                if (i < half) {
+                    const dv2 p = a[2 * u], q = a[2 * u + 1];
                     c0[i] = p;
                     c1[i] = q;
                     v[0] += p.x * p.x + p.y * p.y;
                     v[1] += q.x * q.x + q.y * q.y;
-                } else {
+                }
+            }
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u * g.size;
+                if (i < half) {
+                    const dv2 p = a[2 * u], q = a[2 * u + 1];
                     c0[i] = mk((p.x + 1.0 * q.x) * kInvSqrt2, (p.y + 1.0 * q.y) * kInvSqrt2);
                     c1[i] = mk((p.x + -1.0 * q.x) * kInvSqrt2, (p.y + -1.0 * q.y) * kInvSqrt2);
                     const dv2 s = cadd(p, q), d = csub(p, q);
