@@ -91,9 +91,13 @@ def measured_peaks():
     L = ctypes.CDLL(path)
     L.clifft_probe_lds.restype = ctypes.c_double
     L.clifft_probe_lds.argtypes = [ctypes.c_int, ctypes.c_int]
-    L.clifft_probe_copy.restype = ctypes.c_double
-    L.clifft_probe_copy.argtypes = [ctypes.c_uint64, ctypes.c_int]
-    return {"lds_probe_GBps": L.clifft_probe_lds(256 * 8, 200), "hbm_copy_probe_GBps": L.clifft_probe_copy(4 << 30, 5)}
+    L.clifft_probe_stream.restype = ctypes.c_double
+    L.clifft_probe_stream.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+    # 4 GiB streams, 16384 workgroups of 256 threads, 8 loads in flight per lane
+    return {"lds_probe_GBps": L.clifft_probe_lds(256 * 8, 200),
+            "hbm_read_probe_GBps": L.clifft_probe_stream(0, 4 << 30, 16384, 5),
+            "hbm_write_probe_GBps": L.clifft_probe_stream(1, 4 << 30, 16384, 5),
+            "hbm_copy_probe_GBps": L.clifft_probe_stream(2, 4 << 30, 16384, 5)}
 
 
 def _cpu_model():
