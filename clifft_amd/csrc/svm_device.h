@@ -46,10 +46,11 @@ typedef double dv2 __attribute__((ext_vector_type(2)));
 // Marker (payload bytes 8..9) of a lane-bytecode record parity held inline.
 constexpr uint32_t kLaneInlineParity = 1;
 // Lane-bytecode opcode (outside the reference's range) of a packed run of
-// 2..12 consecutive frame opcodes: axis_1 = ops in the run, axis_2 = the
-// bytecode instructions it replaces minus one (skipped after it); payload
-// words 1..3 hold 4 ops each, 16 bits per op: opcode (bits 0-2), first axis
-// (3-8), second axis (9-14).
+// consecutive frame opcodes: axis_1 = the ops it applies (0..12; the run's
+// opcodes that are the identity on every shot from the zero frame are left
+// out), axis_2 = the bytecode instructions it replaces minus one (skipped
+// after it); payload words 1..3 hold 4 ops each, 16 bits per op: opcode
+// (bits 0-2), first axis (3-8), second axis (9-14).
 constexpr uint32_t kLaneFrameRun = 0xF0;
 constexpr uint32_t kLaneFrameRunMax = 12;
 // Lane-bytecode opcode of a dormant measurement (MEAS_DORMANT_STATIC or
@@ -67,7 +68,7 @@ constexpr uint32_t kLaneDetectorRun = 0xF2;
 struct DevProgram {
     const clifft_instruction* bc;
     const clifft_instruction* lane_bc;  // lane interpreter copy, operands inlined
-    const clifft_instruction* team_bc;  // team interpreter copy: frame runs packed (one-word frames)
+    const clifft_instruction* team_bc;  // team interpreter copy: frame runs packed (one-word frames, zero initial frame)
     const uint64_t* pauli_x;
     const uint64_t* pauli_z;
     const uint8_t* pauli_sign;

@@ -349,38 +349,214 @@ size_t rec_stride_of(const clifft_svm_program* p, bool kfault = false) {
 // Runs of frame opcodes become kLaneFrameRun and runs of inline-parity
 // DETECTORs kLaneDetectorRun (svm_device.h); the instructions they cover
 // stay in place.
-// Runs of 2..kLaneFrameRunMax frame opcodes on axes below 64 become one
-// kLaneFrameRun instruction holding the ops as 16-bit fields (opcode, axis
-// 1, axis 2) in its payload; it names the run's length in axis_1 and the
-// instructions it covers beyond itself in axis_2.  Used by the lane copy and
-// by the team interpreter's copy of one-word-frame programs.
-void pack_frame_runs(clifft_instruction* code, uint64_t n) {
+// CLIFFT_FRAME_PRUNE=0 keeps every frame opcode in the packed copies.
+bool frame_prune_enabled() {
+    static bool v = [] {
+        const char* e = std::getenv("CLIFFT_FRAME_PRUNE");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
+// FRAME_* instructions that are the identity on every shot that starts from
+// the zero frame (every sampled, forced and k-fault shot does).  A forward
+// pass carries, per axis, whether p_x and p_z may be nonzero before each
+// instruction: frame and array Clifford opcodes move and combine those bits
+// as their frame updates do, a measurement sets p_x and clears p_z of its
+// axis, APPLY_PAULI, noise sites and fused U2/U4 nodes may set the bits
+// they name, and any other opcode that writes the frame marks every bit.
+// A FRAME_* opcode whose input bits are all known zero changes neither the
+// frame nor gamma (its handler acts only through those bits: CNOT reads
+// x_c and z_t, CZ x_c and x_t, H x and z, S x, SWAP all four), so it is
+// marked here and left out of its packed run.  No reference counterpart:
+// the reference dispatches every frame opcode.  One-word frames only; a
+// shot that starts from a preset frame runs the unpacked bytecode.
+std::vector<uint8_t> identity_frame_ops(const clifft_program_view* v) {
+    const uint64_t n = v->num_instructions;
+    std::vector<uint8_t> idle(n, 0);
+    if (v->words != 1 || !frame_prune_enabled()) return idle;
+    uint64_t mx = 0, mz = 0;  // bits that may be nonzero before instruction i
+    auto bit = [](uint32_t q) { return q < 64 ? 1ull << q : ~0ull; };
+    auto has = [](uint64_t m, uint32_t q) { return q >= 64 || ((m >> q) & 1ull) != 0; };
+    auto swap_bits = [&](uint64_t& m, uint32_t a, uint32_t b) {
+        const uint64_t ba = bit(a), bb = bit(b);
+        const bool xa = has(m, a), xb = has(m, b);
+        m &= ~(ba | bb);
+        m |= (xb ? ba : 0ull) | (xa ? bb : 0ull);
+    };
+    for (uint64_t i = 0; i < n; ++i) {
+        const clifft_instruction& in = v->bytecode[i];
+        const uint32_t op = in.opcode, a = in.axis_1, b = in.axis_2;
+        const bool frame = op <= CLIFFT_OP_FRAME_SWAP;
+        uint32_t i0, i1;
+        std::memcpy(&i0, in.p.raw, 4);
+        std::memcpy(&i1, in.p.raw + 4, 4);
+        switch (op) {
+            case CLIFFT_OP_FRAME_CNOT:
+            case CLIFFT_OP_ARRAY_CNOT: {  // x_b ^= x_a, z_a ^= z_b
+                const bool xa = has(mx, a), zb = has(mz, b);
+                if (frame && !xa && !zb) idle[i] = 1;
+                if (xa) mx |= bit(b);
+                if (zb) mz |= bit(a);
+                break;
+            }
+            case CLIFFT_OP_FRAME_CZ:
+            case CLIFFT_OP_ARRAY_CZ: {  // z_b ^= x_a, z_a ^= x_b, gamma from x_a x_b
+                const bool xa = has(mx, a), xb = has(mx, b);
+                if (frame && !xa && !xb) idle[i] = 1;
+                if (xa) mz |= bit(b);
+                if (xb) mz |= bit(a);
+                break;
+            }
+            case CLIFFT_OP_FRAME_H:
+            case CLIFFT_OP_ARRAY_H:  // exchanges x_a and z_a, gamma from x_a z_a
+                if (has(mx, a) || has(mz, a)) {
+                    mx |= bit(a);
+                    mz |= bit(a);
+                } else if (frame) {
+                    idle[i] = 1;
+                }
+                break;
+            case CLIFFT_OP_FRAME_S:
+            case CLIFFT_OP_FRAME_S_DAG:
+            case CLIFFT_OP_ARRAY_S:
+            case CLIFFT_OP_ARRAY_S_DAG:  // z_a ^= x_a, gamma from x_a
+                if (has(mx, a))
+                    mz |= bit(a);
+                else if (frame)
+                    idle[i] = 1;
+                break;
+            case CLIFFT_OP_FRAME_SWAP:
+            case CLIFFT_OP_ARRAY_SWAP:
+                if (frame && !has(mx, a) && !has(mx, b) && !has(mz, a) && !has(mz, b)) idle[i] = 1;
+                swap_bits(mx, a, b);
+                swap_bits(mz, a, b);
+                break;
+            case CLIFFT_OP_ARRAY_MULTI_CNOT: {  // target a, controls c: x_a ^= x_c, z_c ^= z_a
+                const uint64_t m = in.p.multi_gate.mask;
+                if (mx & m) mx |= bit(a);
+                if (has(mz, a)) mz |= m;
+                break;
+            }
+            case CLIFFT_OP_ARRAY_MULTI_CZ: {  // z_t ^= x_a, z_a ^= x_t
+                const uint64_t m = in.p.multi_gate.mask;
+                if (has(mx, a)) mz |= m;
+                if (mx & m) mz |= bit(a);
+                break;
+            }
+            case CLIFFT_OP_ARRAY_T:
+            case CLIFFT_OP_ARRAY_T_DAG:
+            case CLIFFT_OP_ARRAY_ROT:
+            case CLIFFT_OP_EXPAND:
+            case CLIFFT_OP_EXPAND_T:
+            case CLIFFT_OP_EXPAND_T_DAG:
+            case CLIFFT_OP_EXPAND_ROT:
+            case CLIFFT_OP_MEAS_DORMANT_STATIC:
+            case CLIFFT_OP_MEAS_DORMANT_STATIC_FORCED:
+            case CLIFFT_OP_READOUT_NOISE:
+            case CLIFFT_OP_DETECTOR:
+            case CLIFFT_OP_POSTSELECT:
+            case CLIFFT_OP_OBSERVABLE:
+            case CLIFFT_OP_EXP_VAL:
+                break;  // read the frame at most
+            case CLIFFT_OP_MEAS_DORMANT_RANDOM:
+            case CLIFFT_OP_MEAS_DORMANT_RANDOM_FORCED:
+            case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL:
+            case CLIFFT_OP_MEAS_ACTIVE_DIAGONAL_FORCED:
+            case CLIFFT_OP_MEAS_ACTIVE_INTERFERE:
+            case CLIFFT_OP_MEAS_ACTIVE_INTERFERE_FORCED:  // x_a = outcome, z_a = 0
+                mx |= bit(a);
+                mz &= ~bit(a);
+                break;
+            case CLIFFT_OP_SWAP_MEAS_INTERFERE:
+            case CLIFFT_OP_SWAP_MEAS_INTERFERE_FORCED:  // frame swap (a, b), then x_b = outcome, z_b = 0
+                if (a != b) {
+                    swap_bits(mx, a, b);
+                    swap_bits(mz, a, b);
+                }
+                mx |= bit(b);
+                mz &= ~bit(b);
+                break;
+            case CLIFFT_OP_ARRAY_U2:
+                mx |= bit(a);
+                mz |= bit(a);
+                break;
+            case CLIFFT_OP_ARRAY_U4:
+                mx |= bit(a) | bit(b);
+                mz |= bit(a) | bit(b);
+                break;
+            case CLIFFT_OP_APPLY_PAULI:
+                if (i0 < v->num_pauli_masks) {
+                    mx |= v->pauli_x[i0];
+                    mz |= v->pauli_z[i0];
+                } else {
+                    mx = mz = ~0ull;
+                }
+                break;
+            case CLIFFT_OP_NOISE:
+            case CLIFFT_OP_NOISE_BLOCK: {
+                const uint64_t s1 = op == CLIFFT_OP_NOISE ? (uint64_t)i0 + 1 : (uint64_t)i0 + i1;
+                if (s1 > v->num_sites) {
+                    mx = mz = ~0ull;
+                    break;
+                }
+                for (uint64_t s = i0; s < s1; ++s)
+                    for (uint32_t c = v->site_chan_off[s]; c < v->site_chan_off[s + 1]; ++c) {
+                        mx |= v->chan_x[c];
+                        mz |= v->chan_z[c];
+                    }
+                break;
+            }
+            default:
+                mx = mz = ~0ull;
+                break;
+        }
+    }
+    return idle;
+}
+
+// Runs of frame opcodes on axes below 64 become one kLaneFrameRun
+// instruction holding the run's non-identity ops (at most
+// kLaneFrameRunMax; `idle` from identity_frame_ops) as 16-bit fields
+// (opcode, axis 1, axis 2) in its payload; it names how many it holds in
+// axis_1 and the instructions it covers beyond itself in axis_2.  A run is
+// packed when it covers two or more instructions or an identity op.  Used
+// by the lane copy and by the team interpreter's copy of one-word-frame
+// programs.
+void pack_frame_runs(clifft_instruction* code, uint64_t n, const std::vector<uint8_t>& idle) {
+    auto frame_op = [&](uint64_t j) {
+        return code[j].opcode <= CLIFFT_OP_FRAME_SWAP && code[j].axis_1 < 64 && code[j].axis_2 < 64;
+    };
     for (uint64_t i = 0; i < n;) {
-        uint64_t j = i;
-        while (j < n && j - i < kLaneFrameRunMax && code[j].opcode <= CLIFFT_OP_FRAME_SWAP && code[j].axis_1 < 64 &&
-               code[j].axis_2 < 64)
+        uint64_t j = i, live = 0;
+        while (j < n && j - i < 0xFFFF && frame_op(j) && (idle[j] || live < kLaneFrameRunMax)) {
+            live += idle[j] ? 0 : 1;
             ++j;
-        if (j - i < 2) {
+        }
+        if (j == i || (j - i == 1 && !idle[i])) {
             i = std::max(j, i + 1);
             continue;
         }
         uint64_t w[3] = {0, 0, 0};
+        uint64_t f_n = 0;
         for (uint64_t t = i; t < j; ++t) {
+            if (idle[t]) continue;
             const uint64_t f = (uint64_t)code[t].opcode | ((uint64_t)code[t].axis_1 << 3) | ((uint64_t)code[t].axis_2 << 9);
-            w[(t - i) / 4] |= f << (16 * ((t - i) % 4));
+            w[f_n / 4] |= f << (16 * (f_n % 4));
+            ++f_n;
         }
         clifft_instruction& h = code[i];
         h.opcode = (uint8_t)kLaneFrameRun;
-        h.axis_1 = (uint16_t)(j - i);
+        h.axis_1 = (uint16_t)f_n;
         h.axis_2 = (uint16_t)(j - i - 1);
         std::memcpy(h.p.raw, w, 24);
         i = j;
     }
 }
 
-void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code) {
+void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code, const std::vector<uint8_t>& idle) {
     const uint64_t n = v->num_instructions;
-    pack_frame_runs(code, n);  // first, so the runs never hold the opcodes below
+    pack_frame_runs(code, n, idle);  // first, so the runs never hold the opcodes below
     for (uint64_t i = 0; i < v->num_instructions; ++i) {
         clifft_instruction& in = code[i];
         uint64_t w2 = 0, w3 = 0;
@@ -961,7 +1137,8 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         // inlined into the unused payload words
         auto* lbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_lbc);
         std::memcpy(lbc, view->bytecode, view->num_instructions * sizeof(clifft_instruction));
-        if (view->words == 1) lane_inline_operands(view, lbc);
+        const std::vector<uint8_t> idle = identity_frame_ops(view);
+        if (view->words == 1) lane_inline_operands(view, lbc, idle);
         auto* lfbc = reinterpret_cast<clifft_instruction*>(staging.data() + o_lfbc);
         std::memcpy(lfbc, lbc, view->num_instructions * sizeof(clifft_instruction));
         for (uint64_t i = 0; i < view->num_instructions; ++i) {
@@ -976,8 +1153,8 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         std::memcpy(tbc, view->bytecode, view->num_instructions * sizeof(clifft_instruction));
         std::memcpy(tfbc, fbc, view->num_instructions * sizeof(clifft_instruction));
         if (view->words == 1 && team_frame_runs_enabled()) {
-            pack_frame_runs(tbc, view->num_instructions);
-            pack_frame_runs(tfbc, view->num_instructions);
+            pack_frame_runs(tbc, view->num_instructions, idle);
+            pack_frame_runs(tfbc, view->num_instructions, idle);
         }
     }
     hipError_t e = hipMalloc(&p->dev_blob, total);

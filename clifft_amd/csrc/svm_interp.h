@@ -627,13 +627,16 @@ struct Interp {
     // ------------------------------------------------------------- dispatch
     // svm_kernels.inl:2183-2646.  Returns when the bytecode ends, a
     // POSTSELECT discards the shot, or a forced outcome is unreachable.
-    // One-word frames read the team copy of the bytecode, where runs of frame
-    // opcodes are packed into kLaneFrameRun instructions (one dispatch per
-    // run; the covered instructions stay in place and are skipped).
-    CSVM_INLINE void run(cptr<DevProgram> Pg, double* ev_out) {
+    // One-word frames that start at zero (zero_frame) read the team copy of
+    // the bytecode, where runs of frame opcodes are packed into kLaneFrameRun
+    // instructions (one dispatch per run; the covered instructions stay in
+    // place and are skipped) without the ops that are the identity from the
+    // zero frame; a preset frame reads the full bytecode.
+    CSVM_INLINE void run(cptr<DevProgram> Pg, double* ev_out, bool zero_frame) {
         constexpr bool kPacked = std::is_same<F, FrameReg<1>>::value;
         cptr<DevProgram> P0 = opaque(Pg);
-        cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(kPacked ? P0->team_bc : P0->bc));
+        const bool packed = kPacked && zero_frame;
+        cptr<uint64_t> code = cp(reinterpret_cast<const uint64_t*>(packed ? P0->team_bc : P0->bc));
         const uint32_t n_instr = P0->n_instr;
         uint32_t skip = 0;
         for (uint32_t pc = 0; pc < n_instr; pc += 1 + skip) {
@@ -645,7 +648,7 @@ struct Interp {
             const uint32_t i0 = (uint32_t)w1, i1 = (uint32_t)(w1 >> 32);
             const uint32_t sign = flags & CLIFFT_FLAG_SIGN;
             skip = 0;
-            if (kPacked && op == kLaneFrameRun) {
+            if (kPacked && op == kLaneFrameRun) {  // never in the full bytecode
                 // frame opcodes keep deferred runs and pending expansions
                 const uint64_t w2 = code[4 * pc + 2], w3 = code[4 * pc + 3];
                 for (uint32_t j = 0; j < a1; ++j) {

@@ -161,7 +161,7 @@ __device__ __forceinline__ void shot_range(cptr<DevProgram> Pg, cptr<ShotArgs> A
                 for (uint32_t i = 0; i < P->num_ev; ++i) ev_row[i] = 0.0;
         }
 
-        S.run(Pg, ev_row);
+        S.run(Pg, ev_row, mode != MODE_PRESET);
 
         A = opaque(Ag);
         P = opaque(Pg);

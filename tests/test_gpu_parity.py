@@ -131,6 +131,44 @@ def test_execute_preset_state(svm, orc, nq, peak, seed):
     assert list(map(int, g.rng)) == list(map(int, o.rng))
 
 
+@pytest.mark.parametrize("nq,t_count", [(12, 5), (20, 9), (24, 14)])
+def test_execute_preset_frame(svm, orc, nq, t_count):
+    """Single-shot execute from a state whose Pauli frame is not zero.  The
+    sampling copies of the bytecode leave out the frame opcodes that are
+    the identity from the zero frame (most of a Clifford+T program's frame
+    opcodes before its first measurement); from this frame they are not, and
+    the device must match the oracle that runs every opcode."""
+    from clifft_amd import workloads as W
+    from clifft_amd.compiler import compile as ccompile
+    from oracle import oracle as orcmod
+    announce("execute-frame", nq, t_count)
+    p = ccompile(W.random_clifford_t(n=nq, layers=20, t_count=t_count, seed=nq + t_count, final_measure=True))
+    frame_ops = np.count_nonzero(p.bytecode["opcode"] <= 5)
+    assert frame_ops >= 50, frame_ops
+    rng = np.random.default_rng(t_count)
+    o = orc.State.for_program(p, seed=3)
+    g = svm.State.for_program(p, seed=3)
+    o.next_noise_idx = 0xFFFFFFFF
+    g.next_noise_idx = 0xFFFFFFFF
+    fx, fz = (int(rng.integers(1, 1 << nq)) for _ in range(2))
+    o.p_x[0] = fx
+    o.p_z[0] = fz
+    g.p_x[0] = fx
+    g.p_z[0] = fz
+    svm.execute(p, g)
+    orcmod.execute(p, o)
+    assert g.active_k == o.active_k
+    n = 1 << g.active_k
+    np.testing.assert_allclose(g.v[:n], o.v[:n], atol=ATOL, rtol=0)
+    assert abs(g.gamma - o.gamma) < ATOL
+    assert int(g.p_x[0]) == int(o.p_x[0]) and int(g.p_z[0]) == int(o.p_z[0])
+    assert np.array_equal(g.meas_record, o.meas_record)
+    # the same program sampled from the zero frame, through the pruned copies
+    res = svm.sample(p, 200, seed=11)
+    m, _, _, _, _ = orc.sample(p, 200, seed=11, per_shot=True)
+    assert np.array_equal(res.measurements, m)
+
+
 @pytest.mark.parametrize("peak", [2, 5])
 def test_survivors_counts(svm, orc, peak):
     announce("survivors", peak)
