@@ -1795,14 +1795,18 @@ clifft_svm_batch_tuning default_tuning() {
     // peak 8, every program of peak 9-16 one shot per wavefront on global
     // amplitudes (no workgroup teams), lanes up to peak 6 -- 5.99e6 shots/s
     // against 4.64e6 for the previous default (workgroup teams with LDS
-    // amplitudes for peak 8-10 and with global ones above 14).
+    // amplitudes for peak 8-10 and with global ones above 14).  Lanes up to
+    // peak 7 (one wavefront per workgroup for the peak-7 columns) measured
+    // later on the same batch: 441.9 against 452.3 ms per step.  A single
+    // program's launch keeps the lane tier to CLIFFT_LANE_MAX_RANK (6): its
+    // shots alone rarely fill the device at one wavefront per workgroup.
     t.global_threads = 256;
     t.wave_rank_max = 8;
     t.lds_rank_max = 8;
     t.wave_chunk = 4;
     t.block_chunk = 16;
     t.concurrent = 1;
-    t.lane_rank_max = lane_max_rank() < 0 ? -1 : std::min(lane_max_rank(), 6);
+    t.lane_rank_max = std::getenv("CLIFFT_LANE_MAX_RANK") ? std::min(lane_max_rank(), 7) : 7;
     t.lane_chunk = 128;
     t.gwave_rank_max = 16;
     return t;
@@ -1815,7 +1819,7 @@ int batch_check_tuning(const clifft_svm_batch_tuning& t) {
         return fail(CLIFFT_E_INVALID, "batch tuning: need wave_rank_max <= lds_rank_max, wave_rank_max <= 10, lds_rank_max <= 13");
     if (t.wave_chunk == 0 || t.block_chunk == 0 || t.lane_chunk == 0)
         return fail(CLIFFT_E_INVALID, "batch tuning: zero chunk");
-    if (t.lane_rank_max > 6) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 6");
+    if (t.lane_rank_max > 7) return fail(CLIFFT_E_INVALID, "batch tuning: lane_rank_max > 7");
     if (t.concurrent > 1) return fail(CLIFFT_E_INVALID, "batch tuning: concurrent must be 0 or 1");
     if (t.gwave_rank_max > kMaxPeakRank) return fail(CLIFFT_E_INVALID, "batch tuning: gwave_rank_max > 30");
     return CLIFFT_OK;
@@ -1828,10 +1832,13 @@ bool batch_shape(const clifft_svm_program* p, const clifft_svm_batch_tuning& t, 
     L.nw = (int)p->nw;
     const uint32_t k = p->peak_rank;
     const size_t red = kTeamScratchBytes, rec = rec_stride_of(p);
-    if (p->words == 1 && (int32_t)k <= t.lane_rank_max &&
-        lane_wave_bytes(k, p->total_slots, p->num_det, p->num_obs) <= kBatchMaxDynLds) {
+    const size_t lane_bytes = lane_wave_bytes(k, p->total_slots, p->num_det, p->num_obs);
+    if (p->words == 1 && (int32_t)k <= t.lane_rank_max && lane_bytes <= kBatchMaxDynLds) {
         L.lane = true;
         L.lane_narrow = p->frame32;
+        // columns too large for two wavefronts per workgroup (peak 7): a
+        // group of its own, so the smaller programs keep their residency
+        if (2 * lane_bytes > kBatchMaxDynLds) L.block = 64;
         return true;
     }
     if (k <= t.wave_rank_max && red + 4 * (rec + ((size_t)16 << k)) <= kBatchMaxDynLds) {
@@ -1859,7 +1866,7 @@ bool batch_shape(const clifft_svm_program* p, const clifft_svm_batch_tuning& t, 
 
 bool same_kernel(const LaunchShape& a, const LaunchShape& b) {
     return a.lane == b.lane && a.lane_narrow == b.lane_narrow && a.nw == b.nw && a.block_team == b.block_team &&
-           a.global_amps == b.global_amps && (a.lane || a.block == b.block);
+           a.global_amps == b.global_amps && a.block == b.block;
 }
 
 bool mono_eligible(const clifft_svm_program* p) {
@@ -2522,7 +2529,7 @@ int clifft_svm_batch_tune(clifft_svm_batch* b, uint64_t sample_shots, void* stre
         }
     };
     if (!rc) try_u32(&clifft_svm_batch_tuning::concurrent, {0});
-    if (!rc) try_lane({-1, 4, 5});
+    if (!rc) try_lane({-1, 4, 5, 6});
     if (!rc) try_u32(&clifft_svm_batch_tuning::lane_chunk, {64, 256, 512, 1024});
     // the LDS thresholds together: which peaks run on wavefront teams in LDS,
     // on workgroup teams in LDS, and (above both) on global amplitudes

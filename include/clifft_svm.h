@@ -369,7 +369,8 @@ int clifft_svm_execute(clifft_svm_program* prog, clifft_svm_state* state);
  * src/clifft/svm/svm.h:302-306).  Many programs of uneven peak rank and
  * bytecode length run as a few persistent launches, one per launch group:
  * the programs that share a kernel shape -- one shot per lane (peak_rank <=
- * lane_rank_max, at most 64 qubits; split by frame-word width), one shot
+ * lane_rank_max, at most 64 qubits; split by frame-word width and by
+ * whether a workgroup holds one wavefront's columns or more), one shot
  * per wavefront (peak_rank <= wave_rank_max), one shot per workgroup with
  * LDS-resident amplitudes (<= lds_rank_max; split by team size), one
  * shot per wavefront with amplitudes in a global slice (<= gwave_rank_max),
@@ -395,7 +396,7 @@ typedef struct {
     uint32_t block_chunk;    /* shots per work item, workgroup teams */
     uint32_t concurrent;     /* 1: launch groups on parallel streams; 0: in sequence */
     int32_t lane_rank_max;   /* <= 64-qubit programs up to this rank run one shot
-                                per lane (-1 = never), <= 6 */
+                                per lane (-1 = never), <= 7 */
     uint32_t lane_chunk;     /* shots per work item, lane tier (a multiple of 64) */
     uint32_t gwave_rank_max; /* global-amplitude programs up to this rank run one shot
                                 per wavefront (<= 30; at or below lds_rank_max: never) */

@@ -117,7 +117,8 @@ TUNINGS = [dict(wave_rank_max=7, lds_rank_max=10, gwave_rank_max=14, wave_chunk=
            dict(lane_rank_max=-1), dict(lane_rank_max=5, lane_chunk=100), dict(lane_rank_max=1, lane_chunk=64),
            dict(lane_rank_max=6, lane_chunk=1024), dict(concurrent=0, lane_rank_max=4, lane_chunk=3000),
            dict(gwave_rank_max=13), dict(gwave_rank_max=16, wave_chunk=3), dict(gwave_rank_max=30, lds_rank_max=8),
-           dict(wave_rank_max=10, lds_rank_max=10), dict(wave_rank_max=6, lds_rank_max=6, gwave_rank_max=16)]
+           dict(wave_rank_max=10, lds_rank_max=10), dict(wave_rank_max=6, lds_rank_max=6, gwave_rank_max=16),
+           dict(lane_rank_max=7), dict(lane_rank_max=7, lane_chunk=64, concurrent=0)]
 
 
 def test_batch_tunings_agree(mods):
@@ -143,6 +144,39 @@ def test_batch_tunings_agree(mods):
                 assert np.array_equal(got[i][j], ref[i][j]), (t, i, j)
             np.testing.assert_allclose(got[i][3], ref[i][3], atol=ATOL, rtol=0)
     b.close()
+
+
+def test_batch_lane_rank7_matches_oracle(mods, orc):
+    """Peak-7 programs one shot per lane (128 KiB of LDS columns, one
+    wavefront per workgroup, a launch group apart from the smaller lane
+    programs), 32-bit and 64-bit frame words, against the oracle."""
+    svm, batch = mods
+    announce("batch-lane-rank7")
+    rng = np.random.default_rng(77)
+    progs, shots, seeds = [], [], []
+    for i in range(8):
+        peak = 7 if i % 4 else 5
+        nq = int(rng.integers(10, 30)) if i % 2 else int(rng.integers(34, 64))
+        progs.append(random_program(7700 + i, num_qubits=nq, peak=peak, length=int(rng.integers(120, 260))))
+        shots.append(int(rng.integers(60, 300)))
+        seeds.append(int(rng.integers(0, 2**63)))
+    assert any(p.peak_rank == 7 and p.num_qubits > 32 for p in progs)
+    assert any(p.peak_rank == 7 and p.num_qubits <= 32 for p in progs)
+    import torch
+    b, bufs = _device_batch(batch, progs, shots)
+    b.set_tuning(lane_rank_max=7)
+    kinds = [g["kind"] for g in b.groups()]
+    assert kinds.count("lane") >= 3, kinds
+    b.run(seeds)
+    torch.cuda.synchronize()
+    got = _snapshot(bufs)
+    b.close()
+    for i, p in enumerate(progs):
+        m, d, o, e, _ = orc.sample(p, shots[i], seed=seeds[i], per_shot=True)
+        assert np.array_equal(got[i][0], m), i
+        assert np.array_equal(got[i][1], d), i
+        assert np.array_equal(got[i][2], o), i
+        np.testing.assert_allclose(got[i][3], e, atol=ATOL, rtol=0)
 
 
 def test_batch_autotune(mods):
@@ -261,6 +295,6 @@ def test_batch_rejects_bad_tuning(mods):
     with pytest.raises(SvmError):
         b.set_tuning(lds_rank_max=14)
     with pytest.raises(SvmError):
-        b.set_tuning(lane_rank_max=7)
+        b.set_tuning(lane_rank_max=8)
     b.run(seeds)  # still usable with the previous tuning
     b.close()
