@@ -64,6 +64,19 @@ constexpr uint32_t kLaneMeasPauli = 0xF1;
 // all inline: the first instruction of the run, axis_2 = run length - 1; the
 // handler reads each detector's own instruction words.
 constexpr uint32_t kLaneDetectorRun = 0xF2;
+// Lane- and team-bytecode opcode of a frame fan: two or more consecutive
+// frame CZs that share qubit p, or CNOTs that share their control or their
+// target p (axis_1 = p, reserved byte = the kind below, payload word 1 = the
+// mask m of the ops' other qubits, axis_2 = the instructions covered minus
+// one).  On the one-word frame (x, z):
+//   kFanCz:      z ^= x_p ? m : 0, z_p ^= parity(x & m), gamma negated when
+//                x_p and parity(x & m) are both 1;
+//   kFanControl: x ^= x_p ? m : 0, z_p ^= parity(z & m);
+//   kFanTarget:  x_p ^= parity(x & m), z ^= z_p ? m : 0.
+// No op of a fan changes a bit another op of it reads, so these equal the
+// ops applied one at a time.
+constexpr uint32_t kLaneFrameFan = 0xF3;
+enum : uint32_t { kFanCz = 0, kFanControl = 1, kFanTarget = 2 };
 
 struct DevProgram {
     const clifft_instruction* bc;

@@ -515,41 +515,112 @@ std::vector<uint8_t> identity_frame_ops(const clifft_program_view* v) {
     return idle;
 }
 
-// Runs of frame opcodes on axes below 64 become one kLaneFrameRun
-// instruction holding the run's non-identity ops (at most
-// kLaneFrameRunMax; `idle` from identity_frame_ops) as 16-bit fields
-// (opcode, axis 1, axis 2) in its payload; it names how many it holds in
-// axis_1 and the instructions it covers beyond itself in axis_2.  A run is
-// packed when it covers two or more instructions or an identity op.  Used
-// by the lane copy and by the team interpreter's copy of one-word-frame
-// programs.
+// Runs of frame opcodes on axes below 64 become superinstructions that
+// cover the run and skip its identity ops (`idle`, from identity_frame_ops):
+// * a fan (kLaneFrameFan) for two or more consecutive non-identity CZs that
+//   share one qubit p, or CNOTs that share their control p or their target
+//   p, held as p and the mask m of their other qubits (an op listed twice
+//   cancels, as the ops do);
+// * a packed run (kLaneFrameRun) for the other ops, at most
+//   kLaneFrameRunMax of them as 16-bit fields (opcode, axis 1, axis 2), the
+//   count in axis_1.
+// Each names the instructions it covers beyond itself in axis_2; together
+// they tile the run.  A single non-identity op covering only itself stays
+// as it is.  Used by the lane copy and by the team interpreter's copy of
+// one-word-frame programs.
 void pack_frame_runs(clifft_instruction* code, uint64_t n, const std::vector<uint8_t>& idle) {
     auto frame_op = [&](uint64_t j) {
         return code[j].opcode <= CLIFFT_OP_FRAME_SWAP && code[j].axis_1 < 64 && code[j].axis_2 < 64;
     };
-    for (uint64_t i = 0; i < n;) {
-        uint64_t j = i, live = 0;
-        while (j < n && j - i < 0xFFFF && frame_op(j) && (idle[j] || live < kLaneFrameRunMax)) {
-            live += idle[j] ? 0 : 1;
-            ++j;
+    // live ops of the fan of `kind` around `pivot` from live[u] on
+    auto fan_len = [&](const std::vector<uint64_t>& live, size_t u, uint32_t kind, uint32_t pivot) {
+        size_t c = 0;
+        for (size_t v = u; v < live.size(); ++v, ++c) {
+            const clifft_instruction& in = code[live[v]];
+            const uint32_t a = in.axis_1, b = in.axis_2;
+            if (a == b) break;
+            if (kind == kFanCz) {
+                if (in.opcode != CLIFFT_OP_FRAME_CZ || (a != pivot && b != pivot)) break;
+            } else if (in.opcode != CLIFFT_OP_FRAME_CNOT || (kind == kFanControl ? a : b) != pivot) {
+                break;
+            }
         }
-        if (j == i || (j - i == 1 && !idle[i])) {
-            i = std::max(j, i + 1);
+        return c;
+    };
+    auto emit_run = [&](uint64_t s, uint64_t e, const std::vector<uint64_t>& ops) {
+        if (e <= s || (e - s == 1 && ops.size() == 1)) return;
+        uint64_t w[3] = {0, 0, 0};
+        for (size_t t = 0; t < ops.size(); ++t) {
+            const clifft_instruction& in = code[ops[t]];
+            const uint64_t f = (uint64_t)in.opcode | ((uint64_t)in.axis_1 << 3) | ((uint64_t)in.axis_2 << 9);
+            w[t / 4] |= f << (16 * (t % 4));
+        }
+        clifft_instruction& h = code[s];
+        h.opcode = (uint8_t)kLaneFrameRun;
+        h.axis_1 = (uint16_t)ops.size();
+        h.axis_2 = (uint16_t)(e - s - 1);
+        std::memcpy(h.p.raw, w, 24);
+    };
+    std::vector<uint64_t> live, ops;
+    for (uint64_t i = 0; i < n;) {
+        uint64_t j = i;
+        while (j < n && j - i < 0xFFFF && frame_op(j)) ++j;
+        if (j == i) {
+            ++i;
             continue;
         }
-        uint64_t w[3] = {0, 0, 0};
-        uint64_t f_n = 0;
-        for (uint64_t t = i; t < j; ++t) {
-            if (idle[t]) continue;
-            const uint64_t f = (uint64_t)code[t].opcode | ((uint64_t)code[t].axis_1 << 3) | ((uint64_t)code[t].axis_2 << 9);
-            w[f_n / 4] |= f << (16 * (f_n % 4));
-            ++f_n;
+        live.clear();
+        for (uint64_t t = i; t < j; ++t)
+            if (!idle[t]) live.push_back(t);
+        uint64_t start = i;  // first instruction of the unit being collected
+        ops.clear();
+        for (size_t u = 0; u < live.size();) {
+            const clifft_instruction& in = code[live[u]];
+            uint32_t kind = 0, pivot = 0;
+            size_t len = 0;
+            if (in.opcode == CLIFFT_OP_FRAME_CZ) {
+                const size_t la = fan_len(live, u, kFanCz, in.axis_1), lb = fan_len(live, u, kFanCz, in.axis_2);
+                kind = kFanCz;
+                pivot = la >= lb ? in.axis_1 : in.axis_2;
+                len = std::max(la, lb);
+            } else if (in.opcode == CLIFFT_OP_FRAME_CNOT) {
+                const size_t lc = fan_len(live, u, kFanControl, in.axis_1), lt = fan_len(live, u, kFanTarget, in.axis_2);
+                kind = lc >= lt ? kFanControl : kFanTarget;
+                pivot = lc >= lt ? in.axis_1 : in.axis_2;
+                len = std::max(lc, lt);
+            }
+            if (len >= 2) {
+                emit_run(start, live[u], ops);
+                ops.clear();
+                uint64_t m = 0;
+                for (size_t v = u; v < u + len; ++v) {
+                    const clifft_instruction& f = code[live[v]];
+                    const uint32_t other = kind == kFanCz ? (f.axis_1 == pivot ? f.axis_2 : f.axis_1)
+                                                          : (kind == kFanControl ? f.axis_2 : f.axis_1);
+                    m ^= 1ull << other;
+                }
+                const uint64_t s = live[u], e = live[u + len - 1] + 1;
+                clifft_instruction& h = code[s];
+                h.opcode = (uint8_t)kLaneFrameFan;
+                h.reserved = (uint8_t)kind;
+                h.flags = 0;
+                h.axis_1 = (uint16_t)pivot;
+                h.axis_2 = (uint16_t)(e - s - 1);
+                std::memset(h.p.raw, 0, 24);
+                std::memcpy(h.p.raw, &m, 8);
+                start = e;
+                u += len;
+                continue;
+            }
+            if (ops.size() == kLaneFrameRunMax) {
+                emit_run(start, live[u], ops);
+                ops.clear();
+                start = live[u];
+            }
+            ops.push_back(live[u]);
+            ++u;
         }
-        clifft_instruction& h = code[i];
-        h.opcode = (uint8_t)kLaneFrameRun;
-        h.axis_1 = (uint16_t)f_n;
-        h.axis_2 = (uint16_t)(j - i - 1);
-        std::memcpy(h.p.raw, w, 24);
+        emit_run(start, j, ops);
         i = j;
     }
 }

@@ -357,6 +357,25 @@ struct Interp {
         f.fz(a, dz);
         f.fz(b, dz);
     }
+    // A fan of frame CZs or CNOTs sharing qubit p (kLaneFrameFan,
+    // svm_device.h): the ops one at a time, on the frame's first word.
+    CSVM_INLINE void frame_fan(uint32_t kind, uint32_t p, uint64_t m) {
+        const uint64_t x = f.wx(0), z = f.wz(0), pb = 1ull << p;
+        if (kind == kFanCz) {
+            const uint64_t xp = (x >> p) & 1ull;
+            const uint32_t par = (uint32_t)__popcll(x & m) & 1u;
+            if (xp && par) negate_gamma();
+            f.xor_words(0, 0ull, (xp ? m : 0ull) ^ (par ? pb : 0ull));
+        } else if (kind == kFanControl) {
+            const uint64_t xp = (x >> p) & 1ull;
+            const uint32_t par = (uint32_t)__popcll(z & m) & 1u;
+            f.xor_words(0, xp ? m : 0ull, par ? pb : 0ull);
+        } else {
+            const uint64_t zp = (z >> p) & 1ull;
+            const uint32_t par = (uint32_t)__popcll(x & m) & 1u;
+            f.xor_words(0, par ? pb : 0ull, zp ? m : 0ull);
+        }
+    }
     CSVM_INLINE void set_x(uint32_t q, uint32_t b) { f.fx(q, f.gx(q) ^ b); }
     CSVM_INLINE void set_z(uint32_t q, uint32_t b) { f.fz(q, f.gz(q) ^ b); }
 
@@ -629,9 +648,10 @@ struct Interp {
     // POSTSELECT discards the shot, or a forced outcome is unreachable.
     // One-word frames that start at zero (zero_frame) read the team copy of
     // the bytecode, where runs of frame opcodes are packed into kLaneFrameRun
-    // instructions (one dispatch per run; the covered instructions stay in
-    // place and are skipped) without the ops that are the identity from the
-    // zero frame; a preset frame reads the full bytecode.
+    // and kLaneFrameFan instructions (one dispatch per unit; the covered
+    // instructions stay in place and are skipped) without the ops that are
+    // the identity from the zero frame; a preset frame reads the full
+    // bytecode.
     CSVM_INLINE void run(cptr<DevProgram> Pg, double* ev_out, bool zero_frame) {
         constexpr bool kPacked = std::is_same<F, FrameReg<1>>::value;
         cptr<DevProgram> P0 = opaque(Pg);
@@ -662,6 +682,11 @@ struct Interp {
                     else if (fop == CLIFFT_OP_FRAME_S_DAG) frame_s(fa, -1.0);
                     else frame_swap(fa, fb);
                 }
+                skip = a2;
+                continue;
+            }
+            if (kPacked && op == kLaneFrameFan) {  // never in the full bytecode
+                frame_fan((uint32_t)(w0 >> 8) & 0xFFu, a1, w1);
                 skip = a2;
                 continue;
             }

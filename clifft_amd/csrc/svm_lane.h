@@ -356,6 +356,26 @@ struct LaneInterp {
                         else frame_s(fa);
                     }
                     break;
+                case kLaneFrameFan: {  // frame CZs or CNOTs sharing qubit a1 (svm_device.h)
+                    const FW m = (FW)w1, pb = (FW)1 << a1;
+                    const uint32_t kind = (uint32_t)(w0 >> 8) & 0xFFu;
+                    if (kind == kFanCz) {
+                        const FW xp = (px >> a1) & 1u;
+                        const FW par = (FW)(__popcll((uint64_t)(px & m)) & 1);
+                        pz ^= (xp ? m : (FW)0) ^ (par ? pb : (FW)0);
+                    } else if (kind == kFanControl) {
+                        const FW xp = (px >> a1) & 1u;
+                        const FW par = (FW)(__popcll((uint64_t)(pz & m)) & 1);
+                        px ^= xp ? m : (FW)0;
+                        pz ^= par ? pb : (FW)0;
+                    } else {
+                        const FW zp = (pz >> a1) & 1u;
+                        const FW par = (FW)(__popcll((uint64_t)(px & m)) & 1);
+                        px ^= par ? pb : (FW)0;
+                        pz ^= zp ? m : (FW)0;
+                    }
+                    break;
+                }
                 case kLaneDetectorRun:  // DETECTORs with inline parities, each from its own words
                     for (uint32_t j = 0; j <= a2; ++j) {
                         const uint64_t x0 = code[4 * (pc + j)], x1 = code[4 * (pc + j) + 1];
