@@ -76,6 +76,12 @@ constexpr uint32_t kLaneDetectorRun = 0xF2;
 // No op of a fan changes a bit another op of it reads, so these equal the
 // ops applied one at a time.
 constexpr uint32_t kLaneFrameFan = 0xF3;
+// Team-bytecode opcode of two or more consecutive ARRAY_CNOTs that share
+// their control (axis_1), payload word 1 = the mask of their targets (a
+// target listed twice cancels), axis_2 = the instructions covered minus one.
+// The CNOTs commute; the amplitude permutation is x ^= mask where bit
+// axis_1 of x is set, and the frame update a kFanControl fan.
+constexpr uint32_t kArrayFanout = 0xF4;
 enum : uint32_t { kFanCz = 0, kFanControl = 1, kFanTarget = 2 };
 
 struct DevProgram {
@@ -212,7 +218,15 @@ constexpr uint64_t kMonoMeasOps =
 // qm = b, pk 1.  MULTI_CZ(a, m): qa = a, qm = m, pk 1.  S/S_DAG/T/T_DAG/ROT
 // on a: qa = qm = a, pk 2 (masks as single bits).  kind, a and b name the
 // opcode for the single-opcode sweeps.
-enum : uint32_t { MONO_CNOT = 0, MONO_CZ = 1, MONO_SWAP = 2, MONO_MULTI_CNOT = 3, MONO_MULTI_CZ = 4, MONO_PHASE = 5 };
+enum : uint32_t {
+    MONO_CNOT = 0,
+    MONO_CZ = 1,
+    MONO_SWAP = 2,
+    MONO_MULTI_CNOT = 3,
+    MONO_MULTI_CZ = 4,
+    MONO_PHASE = 5,
+    MONO_FANOUT = 6  // CNOTs from control a to every target in mask m
+};
 struct MonoOp {
     uint32_t kind;
     uint32_t a;  // control / first axis / MULTI axis / phase axis

@@ -625,6 +625,46 @@ void pack_frame_runs(clifft_instruction* code, uint64_t n, const std::vector<uin
     }
 }
 
+// CLIFFT_ARRAY_FANOUT=0 keeps consecutive ARRAY_CNOTs one instruction each.
+bool array_fanout_enabled() {
+    static bool v = [] {
+        const char* e = std::getenv("CLIFFT_ARRAY_FANOUT");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
+// Two or more consecutive ARRAY_CNOTs with the same control and targets other
+// than it become one kArrayFanout instruction (control, target mask) that
+// covers them: one deferred permutation opcode instead of one per CNOT.
+// Team copy of one-word-frame programs only.
+void pack_array_fanouts(clifft_instruction* code, uint64_t n) {
+    for (uint64_t i = 0; i < n;) {
+        const clifft_instruction& h0 = code[i];
+        if (h0.opcode != CLIFFT_OP_ARRAY_CNOT || h0.axis_1 == h0.axis_2 || h0.axis_1 >= 32 || h0.axis_2 >= 32) {
+            ++i;
+            continue;
+        }
+        const uint32_t c = h0.axis_1;
+        uint64_t j = i;
+        uint32_t m = 0;
+        while (j < n && j - i < 0xFFFF && code[j].opcode == CLIFFT_OP_ARRAY_CNOT && code[j].axis_1 == c &&
+               code[j].axis_2 != c && code[j].axis_2 < 32) {
+            m ^= 1u << code[j].axis_2;
+            ++j;
+        }
+        if (j - i >= 2) {
+            clifft_instruction& h = code[i];
+            h.opcode = (uint8_t)kArrayFanout;
+            h.axis_2 = (uint16_t)(j - i - 1);
+            std::memset(h.p.raw, 0, 24);
+            const uint64_t m64 = m;
+            std::memcpy(h.p.raw, &m64, 8);
+        }
+        i = std::max(j, i + 1);
+    }
+}
+
 void lane_inline_operands(const clifft_program_view* v, clifft_instruction* code, const std::vector<uint8_t>& idle) {
     const uint64_t n = v->num_instructions;
     pack_frame_runs(code, n, idle);  // first, so the runs never hold the opcodes below
@@ -1226,6 +1266,10 @@ int clifft_svm_program_create(const clifft_program_view* view, clifft_svm_progra
         if (view->words == 1 && team_frame_runs_enabled()) {
             pack_frame_runs(tbc, view->num_instructions, idle);
             pack_frame_runs(tfbc, view->num_instructions, idle);
+            if (array_fanout_enabled()) {
+                pack_array_fanouts(tbc, view->num_instructions);
+                pack_array_fanouts(tfbc, view->num_instructions);
+            }
         }
     }
     hipError_t e = hipMalloc(&p->dev_blob, total);

@@ -177,6 +177,9 @@ struct Interp {
             if (kind == MONO_CNOT) {
                 o.pm = 1u << a;
                 o.px = 1u << b;
+            } else if (kind == MONO_FANOUT) {
+                o.pm = 1u << a;
+                o.px = m;
             } else if (kind == MONO_SWAP) {
                 o.pm = o.px = (1u << a) | (1u << b);
             } else if (kind == MONO_MULTI_CNOT) {
@@ -195,13 +198,14 @@ struct Interp {
         ++nmono;
         return true;
     }
-    // Apply the deferred run: a single opcode in place with its own sweep,
-    // two or more in one out-of-place pass into the other slice.  The run
+    // Apply the deferred run: a single opcode in place with its own sweep
+    // (a fan-out through the out-of-place pass), two or more in one
+    // out-of-place pass into the other slice.  The run
     // was recorded over the expanded state: no expansion may be pending.
     CSVM_INLINE void flush_mono() {
         if (!kGlobalAmps || nmono == 0) return;
         team_sync<BT>();  // the list written by lane 0 is visible to every wave
-        if (nmono == 1) {
+        if (nmono == 1 && uni(mops->kind) != MONO_FANOUT) {
             const uint32_t kind = uni(mops->kind), a = uni(mops->a), b = uni(mops->b);
             if (kind == MONO_CNOT) sweep_two_axis<BT>(amp, k, a, b, PERM_CNOT, g);
             else if (kind == MONO_CZ) sweep_two_axis<BT>(amp, k, a, b, PERM_CZ, g);
@@ -695,9 +699,11 @@ struct Interp {
             // the expansions stay pending under runs and up to that
             // measurement.  Expansions are written before the run recorded
             // over them.
-            const bool mflush = nmono && !((kMonoKeepOps >> op) & 1ull) &&
-                                !((nmono >= 2 || npend || amp_alt2) && ((kMonoMeasOps >> op) & 1ull));
-            if (npend && (mflush || (!((kNoAmpOps >> op) & 1ull) && !(pend_defer && (((kMonoOps | kMonoMeasOps) >> op) & 1ull)))))
+            // a fan-out of ARRAY_CNOTs flushes what one ARRAY_CNOT would
+            const uint32_t opk = (kPacked && op == kArrayFanout) ? (uint32_t)CLIFFT_OP_ARRAY_CNOT : op;
+            const bool mflush = nmono && !((kMonoKeepOps >> opk) & 1ull) &&
+                                !((nmono >= 2 || npend || amp_alt2) && ((kMonoMeasOps >> opk) & 1ull));
+            if (npend && (mflush || (!((kNoAmpOps >> opk) & 1ull) && !(pend_defer && (((kMonoOps | kMonoMeasOps) >> opk) & 1ull)))))
                 flush_expand_oldest(npend);
             if (mflush) flush_mono();
             switch (op) {
@@ -711,6 +717,15 @@ struct Interp {
                     if (!defer_mono(MONO_CNOT, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_CNOT, g);
                     frame_cnot(a1, a2);
                     break;
+                case kArrayFanout: {  // ARRAY_CNOTs from control a1 to the targets in w1
+                    const uint32_t t = (uint32_t)w1;
+                    if (!defer_mono(MONO_FANOUT, a1, 0u, t, 0.0, 0.0))
+                        for (uint32_t r = t; r; r &= r - 1u)
+                            sweep_two_axis<BT>(amp, k, a1, (uint32_t)__builtin_ctz(r), PERM_CNOT, g);
+                    frame_fan(kFanControl, a1, (uint64_t)t);
+                    skip = a2;
+                    break;
+                }
                 case CLIFFT_OP_ARRAY_CZ:  // :255-365
                     if (!defer_mono(MONO_CZ, a1, a2, 0u, 0.0, 0.0)) sweep_two_axis<BT>(amp, k, a1, a2, PERM_CZ, g);
                     frame_cz(a1, a2);

@@ -145,7 +145,7 @@ def _pack_text() -> str:
     with open(os.path.join(ROOT, "clifft_amd", "csrc", "svm_host.cpp")) as f:
         src = f.read()
     body = src[src.index("// Runs of frame opcodes on axes below 64 become superinstructions"):
-               src.index("void lane_inline_operands(")]
+               src.index("// CLIFFT_ARRAY_FANOUT=0 keeps")]
     return "\n".join(consts) + "\n" + body
 
 
