@@ -76,11 +76,13 @@ constexpr uint32_t kLaneDetectorRun = 0xF2;
 // No op of a fan changes a bit another op of it reads, so these equal the
 // ops applied one at a time.
 constexpr uint32_t kLaneFrameFan = 0xF3;
-// Team-bytecode opcode of two or more consecutive ARRAY_CNOTs that share
-// their control (axis_1), payload word 1 = the mask of their targets (a
-// target listed twice cancels), axis_2 = the instructions covered minus one.
-// The CNOTs commute; the amplitude permutation is x ^= mask where bit
-// axis_1 of x is set, and the frame update a kFanControl fan.
+// Team-bytecode opcode of consecutive ARRAY_CNOTs that share their control
+// (axis_1), payload word 1 = the mask of their targets (a target listed
+// twice cancels), axis_2 = the instructions covered minus one.  The CNOTs
+// commute; the amplitude permutation is x ^= mask where bit axis_1 of x is
+// set, and the frame update a kFanControl fan.  Reserved byte 1: the
+// ARRAY_MULTI_CZ from the same control that follows them is covered too,
+// its mask in payload word 2 (two or more CNOTs, or one and the MULTI_CZ).
 constexpr uint32_t kArrayFanout = 0xF4;
 enum : uint32_t { kFanCz = 0, kFanControl = 1, kFanTarget = 2 };
 
@@ -225,7 +227,8 @@ enum : uint32_t {
     MONO_MULTI_CNOT = 3,
     MONO_MULTI_CZ = 4,
     MONO_PHASE = 5,
-    MONO_FANOUT = 6  // CNOTs from control a to every target in mask m
+    MONO_FANOUT = 6,    // CNOTs from control a to every target in mask m
+    MONO_FANOUT_CZ = 7  // the same, then a MULTI_CZ from a (phase part qa, qm)
 };
 struct MonoOp {
     uint32_t kind;

@@ -636,8 +636,10 @@ bool array_fanout_enabled() {
 
 // Two or more consecutive ARRAY_CNOTs with the same control and targets other
 // than it become one kArrayFanout instruction (control, target mask) that
-// covers them: one deferred permutation opcode instead of one per CNOT.
-// Team copy of one-word-frame programs only.
+// covers them: one deferred permutation opcode instead of one per CNOT.  An
+// ARRAY_MULTI_CZ from the same control right after them (not targeting the
+// control) is covered too, also after a single CNOT.  Team copy of
+// one-word-frame programs only.
 void pack_array_fanouts(clifft_instruction* code, uint64_t n) {
     for (uint64_t i = 0; i < n;) {
         const clifft_instruction& h0 = code[i];
@@ -653,13 +655,20 @@ void pack_array_fanouts(clifft_instruction* code, uint64_t n) {
             m ^= 1u << code[j].axis_2;
             ++j;
         }
-        if (j - i >= 2) {
+        uint64_t cz = 0;
+        const bool with_cz = j < n && j - i < 0xFFFF && code[j].opcode == CLIFFT_OP_ARRAY_MULTI_CZ &&
+                             code[j].axis_1 == c && ((code[j].p.multi_gate.mask >> c) & 1ull) == 0;
+        if (with_cz) cz = code[j].p.multi_gate.mask;
+        if (j - i >= 2 || with_cz) {
+            if (with_cz) ++j;
             clifft_instruction& h = code[i];
             h.opcode = (uint8_t)kArrayFanout;
+            h.reserved = with_cz ? 1u : 0u;
             h.axis_2 = (uint16_t)(j - i - 1);
             std::memset(h.p.raw, 0, 24);
             const uint64_t m64 = m;
             std::memcpy(h.p.raw, &m64, 8);
+            std::memcpy(h.p.raw + 8, &cz, 8);
         }
         i = std::max(j, i + 1);
     }

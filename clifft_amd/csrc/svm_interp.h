@@ -177,9 +177,14 @@ struct Interp {
             if (kind == MONO_CNOT) {
                 o.pm = 1u << a;
                 o.px = 1u << b;
-            } else if (kind == MONO_FANOUT) {
+            } else if (kind == MONO_FANOUT || kind == MONO_FANOUT_CZ) {
                 o.pm = 1u << a;
                 o.px = m;
+                if (kind == MONO_FANOUT_CZ) {  // the phase, as seen before the permutation
+                    o.qa = 1u << a;
+                    o.qm = b;
+                    o.pk = 1u;
+                }
             } else if (kind == MONO_SWAP) {
                 o.pm = o.px = (1u << a) | (1u << b);
             } else if (kind == MONO_MULTI_CNOT) {
@@ -205,7 +210,7 @@ struct Interp {
     CSVM_INLINE void flush_mono() {
         if (!kGlobalAmps || nmono == 0) return;
         team_sync<BT>();  // the list written by lane 0 is visible to every wave
-        if (nmono == 1 && uni(mops->kind) != MONO_FANOUT) {
+        if (nmono == 1 && uni(mops->kind) < MONO_FANOUT) {
             const uint32_t kind = uni(mops->kind), a = uni(mops->a), b = uni(mops->b);
             if (kind == MONO_CNOT) sweep_two_axis<BT>(amp, k, a, b, PERM_CNOT, g);
             else if (kind == MONO_CZ) sweep_two_axis<BT>(amp, k, a, b, PERM_CZ, g);
@@ -719,10 +724,19 @@ struct Interp {
                     break;
                 case kArrayFanout: {  // ARRAY_CNOTs from control a1 to the targets in w1
                     const uint32_t t = (uint32_t)w1;
-                    if (!defer_mono(MONO_FANOUT, a1, 0u, t, 0.0, 0.0))
+                    // then the covered MULTI_CZ from a1 (:559-635), on the active axes
+                    const bool cz = ((w0 >> 8) & 0xFFu) != 0u;
+                    const uint64_t mz = cz ? (code[4 * pc + 2] & ((1ULL << k) - 1ULL)) : 0ull;
+                    // its sign condition x_a1 & parity(y & mz) at y = the permuted
+                    // index, as a condition on the index before the permutation
+                    const uint32_t qm = (uint32_t)mz ^ ((__popcll((uint64_t)t & mz) & 1) ? (1u << a1) : 0u);
+                    if (!defer_mono(cz ? MONO_FANOUT_CZ : MONO_FANOUT, a1, qm, t, 0.0, 0.0)) {
                         for (uint32_t r = t; r; r &= r - 1u)
                             sweep_two_axis<BT>(amp, k, a1, (uint32_t)__builtin_ctz(r), PERM_CNOT, g);
+                        if (cz) sweep_multi<BT>(amp, k, a1, (uint32_t)mz, 1u, g);
+                    }
                     frame_fan(kFanControl, a1, (uint64_t)t);
+                    if (cz) frame_fan(kFanCz, a1, mz);
                     skip = a2;
                     break;
                 }
