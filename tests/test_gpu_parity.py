@@ -105,6 +105,28 @@ def test_record_log_probs(svm, orc, nq, peak):
     np.testing.assert_allclose(gpu[~unreach_g], ref[~unreach_r], atol=1e-10, rtol=1e-12)
 
 
+@pytest.mark.parametrize("nq,t_count", [(20, 9), (24, 14)])
+def test_record_log_probs_compiled(svm, orc, nq, t_count):
+    """Forced execution of compiled Clifford+T programs, whose final blocks
+    hold frame fans and array fan-outs (with the MULTI_CZ that follows them)
+    in the forced copy of the team bytecode: log-probabilities of sampled
+    records, and of records with one flipped outcome, against the oracle."""
+    from clifft_amd import workloads as W
+    from clifft_amd.compiler import compile as ccompile
+    announce("forced-compiled", nq, t_count)
+    p = ccompile(W.random_clifford_t(n=nq, layers=20, t_count=t_count, seed=nq * 3 + t_count, final_measure=True))
+    recs, *_ = orc.sample(p, 24, seed=5)
+    rng = np.random.default_rng(t_count)
+    flipped = recs.copy()
+    flipped[::2, rng.integers(0, p.num_measurements)] ^= 1
+    allrec = np.concatenate([recs, flipped])
+    gpu = svm.record_probabilities(p, allrec)
+    ref = orc.record_log_probs(p.forced(), allrec)
+    unreach_g, unreach_r = gpu < -1e300, ref < -1e300
+    assert np.array_equal(unreach_g, unreach_r)
+    np.testing.assert_allclose(gpu[~unreach_g], ref[~unreach_r], atol=1e-10, rtol=1e-12)
+
+
 @pytest.mark.parametrize("nq,peak", [(10, 5), (16, 9), (18, 14)])
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_execute_preset_state(svm, orc, nq, peak, seed):
