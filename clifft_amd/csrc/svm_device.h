@@ -81,8 +81,9 @@ constexpr uint32_t kLaneFrameFan = 0xF3;
 // twice cancels), axis_2 = the instructions covered minus one.  The CNOTs
 // commute; the amplitude permutation is x ^= mask where bit axis_1 of x is
 // set, and the frame update a kFanControl fan.  Reserved byte 1: the
-// ARRAY_MULTI_CZ from the same control that follows them is covered too,
-// its mask in payload word 2 (two or more CNOTs, or one and the MULTI_CZ).
+// ARRAY_MULTI_CZ from the same control (or ARRAY_CZ on it) that follows them
+// is covered too, its target mask in payload word 2 (two or more CNOTs, or
+// one and the CZ).
 constexpr uint32_t kArrayFanout = 0xF4;
 enum : uint32_t { kFanCz = 0, kFanControl = 1, kFanTarget = 2 };
 

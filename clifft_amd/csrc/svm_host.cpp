@@ -638,8 +638,8 @@ bool array_fanout_enabled() {
 // than it become one kArrayFanout instruction (control, target mask) that
 // covers them: one deferred permutation opcode instead of one per CNOT.  An
 // ARRAY_MULTI_CZ from the same control right after them (not targeting the
-// control) is covered too, also after a single CNOT.  Team copy of
-// one-word-frame programs only.
+// control), or an ARRAY_CZ on the control, is covered too, also after a
+// single CNOT.  Team copy of one-word-frame programs only.
 void pack_array_fanouts(clifft_instruction* code, uint64_t n) {
     for (uint64_t i = 0; i < n;) {
         const clifft_instruction& h0 = code[i];
@@ -656,9 +656,18 @@ void pack_array_fanouts(clifft_instruction* code, uint64_t n) {
             ++j;
         }
         uint64_t cz = 0;
-        const bool with_cz = j < n && j - i < 0xFFFF && code[j].opcode == CLIFFT_OP_ARRAY_MULTI_CZ &&
-                             code[j].axis_1 == c && ((code[j].p.multi_gate.mask >> c) & 1ull) == 0;
-        if (with_cz) cz = code[j].p.multi_gate.mask;
+        bool with_cz = false;
+        if (j < n && j - i < 0xFFFF) {
+            const clifft_instruction& z = code[j];
+            if (z.opcode == CLIFFT_OP_ARRAY_MULTI_CZ && z.axis_1 == c && ((z.p.multi_gate.mask >> c) & 1ull) == 0) {
+                with_cz = true;
+                cz = z.p.multi_gate.mask;
+            } else if (z.opcode == CLIFFT_OP_ARRAY_CZ && z.axis_1 != z.axis_2 && (z.axis_1 == c || z.axis_2 == c) &&
+                       z.axis_1 < 32 && z.axis_2 < 32) {
+                with_cz = true;  // a CZ is a MULTI_CZ with one target
+                cz = 1ull << (z.axis_1 == c ? z.axis_2 : z.axis_1);
+            }
+        }
         if (j - i >= 2 || with_cz) {
             if (with_cz) ++j;
             clifft_instruction& h = code[i];
